@@ -1,0 +1,277 @@
+"""Python binding of libspotfinder_hip.so, the gfx950 spot-finding backend.
+
+A thin ctypes layer over the C ABI in include/spotfinder_hip.h: the tests and
+the benchmark call the kernels through exactly the surface a C, C++ or
+foreign-language caller would.  Loading fails loudly if the library was not
+built (``make -C dials-metal_amd`` or ``__graft_entry__.build()``); there is no
+CPU fallback.
+
+Reference interfaces mirrored (paths in the reference repository):
+  find                 gpu::find<T>            src/spots/dext_gpu.hh:171-180
+  available, ...       gpu::available, ...     src/spots/dext_gpu.hh:137-154
+  Labeller             dials_spots::Labeller   src/spots/dials_spots.hh:108-158
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIBRARY = os.path.join(HERE, "libspotfinder_hip.so")
+
+SIGNAL_PIXEL = np.dtype(
+    [("index", "<u4"), ("value", "<u4"), ("background", "<f4"),
+     ("population", "<u2"), ("reserved", "<u2")])
+PIXEL = np.dtype([("frame", "<i4"), ("index", "<u4"), ("value", "<u4")])
+SPOT = np.dtype(
+    [("bbox", "<i4", (6,)), ("position", "<f8", (3,)), ("variance", "<f8", (3,)),
+     ("intensity", "<f8"), ("intensity_variance", "<f8"), ("n_signal", "<u4"),
+     ("first", "<u8")], align=True)
+COUNTS_FIELDS = ("signal_pixels", "groups", "too_small", "too_large",
+                 "separated", "accepted")
+
+KERNELS = {"fused": 0, "direct": 1}
+
+
+class SpotfinderError(RuntimeError):
+    """A status other than success from the C ABI."""
+
+    def __init__(self, status, message):
+        super().__init__(f"status {status}: {message}")
+        self.status = status
+
+
+class StageTimes(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_double)
+                for f in ("upload", "stage0", "stage1", "stage2", "fused",
+                          "copy", "sort")]
+
+
+class SynthDesc(ctypes.Structure):
+    _fields_ = [("width", ctypes.c_uint32), ("height", ctypes.c_uint32),
+                ("bits", ctypes.c_uint32),
+                ("module_w", ctypes.c_uint32), ("gap_w", ctypes.c_uint32),
+                ("module_h", ctypes.c_uint32), ("gap_h", ctypes.c_uint32),
+                ("seed", ctypes.c_uint64),
+                ("background", ctypes.c_double), ("ring_mean", ctypes.c_double),
+                ("ring_radius", ctypes.c_double), ("ring_sigma", ctypes.c_double),
+                ("spots_per_frame", ctypes.c_uint32),
+                ("sigma_min", ctypes.c_double), ("sigma_max", ctypes.c_double),
+                ("peak_min", ctypes.c_double), ("peak_max", ctypes.c_double),
+                ("hot_fraction", ctypes.c_double),
+                ("persist", ctypes.c_uint32), ("threads", ctypes.c_uint32)]
+
+
+_lib = None
+
+
+def lib():
+    """The loaded library.  Raises if it has not been built."""
+    global _lib
+    if _lib is not None:
+        return _lib
+    if not os.path.exists(LIBRARY):
+        raise ImportError(
+            f"{LIBRARY} is not built: run `make -C dials-metal_amd` or "
+            "__graft_entry__.build(); there is no CPU fallback")
+    L = ctypes.CDLL(LIBRARY)
+    vp, sz, u32 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint32
+    sigs = {
+        "sf_backend": ([], ctypes.c_char_p),
+        "sf_available": ([], ctypes.c_int),
+        "sf_unavailable_reason": ([], ctypes.c_char_p),
+        "sf_memory_free": ([], sz),
+        "sf_host_alloc": ([sz], vp),
+        "sf_host_free": ([vp], None),
+        "sf_last_error": ([], ctypes.c_char_p),
+        "sf_set_kernel": ([ctypes.c_int], None),
+        "sf_get_kernel": ([], ctypes.c_int),
+        "sf_profile_stages": ([ctypes.c_int], None),
+        "sf_profiling_stages": ([], ctypes.c_int),
+        "sf_last_stage_times": ([], StageTimes),
+        "sf_find_run": ([ctypes.c_int, vp, sz, sz, ctypes.POINTER(sz)], ctypes.c_int),
+        "sf_find_fetch": ([vp, sz], ctypes.c_int),
+        "sf_ws_create": ([sz, vp, ctypes.POINTER(vp)], ctypes.c_int),
+        "sf_ws_destroy": ([vp], None),
+        "sf_ws_stream": ([vp], vp),
+        "sf_ws_threshold": ([vp, ctypes.c_int, vp, u32, u32, ctypes.c_int], ctypes.c_int),
+        "sf_ws_sync": ([vp, ctypes.POINTER(sz)], ctypes.c_int),
+        "sf_ws_fetch": ([vp, vp, sz], ctypes.c_int),
+        "sf_ws_device_signal": ([vp], vp),
+        "sf_ws_set_timing": ([vp, ctypes.c_int], None),
+        "sf_ws_kernel_ms": ([vp, ctypes.POINTER(ctypes.c_double),
+                             ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "sf_ws_fallbacks": ([vp], ctypes.c_uint64),
+        "sf_ws_device_mask": ([vp, ctypes.POINTER(u32)], vp),
+        "sf_synth_default": ([ctypes.POINTER(SynthDesc), u32, u32], None),
+        "sf_synth_eiger": ([ctypes.POINTER(SynthDesc), ctypes.c_int, ctypes.c_int], None),
+        "sf_synth_frame": ([ctypes.POINTER(SynthDesc), ctypes.c_int64, vp], ctypes.c_int),
+    }
+    for name, (args, res) in sigs.items():
+        f = getattr(L, name)
+        f.argtypes = args
+        f.restype = res
+    _lib = L
+    return L
+
+
+def _ptr(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+def last_error():
+    return lib().sf_last_error().decode()
+
+
+def _check(status):
+    if status != 0:
+        raise SpotfinderError(status, last_error())
+
+
+def backend():
+    return lib().sf_backend().decode()
+
+
+def available():
+    return bool(lib().sf_available())
+
+
+def unavailable_reason():
+    return lib().sf_unavailable_reason().decode()
+
+
+def memory_free():
+    return int(lib().sf_memory_free())
+
+
+def set_kernel(name):
+    lib().sf_set_kernel(KERNELS[name])
+
+
+def get_kernel():
+    k = lib().sf_get_kernel()
+    return {v: n for n, v in KERNELS.items()}[k]
+
+
+def profile_stages(on):
+    lib().sf_profile_stages(int(bool(on)))
+
+
+def last_stage_times():
+    t = lib().sf_last_stage_times()
+    return {f: getattr(t, f) for f, _ in StageTimes._fields_}
+
+
+def find(frame):
+    """gpu::find<T>: (status, SignalPixel array ascending by index) for a
+    host uint16 or uint32 frame.  Status as dext(): 0, -1 or -2; a device
+    failure raises SpotfinderError."""
+    frame = np.ascontiguousarray(frame)
+    if frame.dtype == np.uint16:
+        bits = 16
+    elif frame.dtype == np.uint32:
+        bits = 32
+    else:
+        raise TypeError(f"no threshold for {frame.dtype}")
+    if frame.ndim != 2:
+        raise ValueError("a frame is two-dimensional")
+    h, w = frame.shape
+    found = ctypes.c_size_t(0)
+    rc = lib().sf_find_run(bits, _ptr(frame) if frame.size else None, h, w,
+                           ctypes.byref(found))
+    if rc == -3 or rc == -4:
+        _check(rc)
+    if rc != 0:
+        return rc, np.zeros(0, SIGNAL_PIXEL)
+    out = np.zeros(found.value, SIGNAL_PIXEL)
+    _check(lib().sf_find_fetch(_ptr(out), found.value))
+    return 0, out
+
+
+class Workspace:
+    """Device-resident thresholding (sf_ws_*): frames already in HBM, results
+    kept there.  `stream` is a hipStream_t (int) to borrow, or None."""
+
+    def __init__(self, stream=None):
+        self.h = ctypes.c_void_p()
+        _check(lib().sf_ws_create(0, stream, ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().sf_ws_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    @property
+    def stream(self):
+        return lib().sf_ws_stream(self.h)
+
+    def threshold(self, dev_ptr, bits, height, width, kernel="fused"):
+        _check(lib().sf_ws_threshold(self.h, bits, dev_ptr, height, width,
+                                     KERNELS[kernel]))
+
+    def sync(self):
+        """(status, count): status 0 or -2."""
+        found = ctypes.c_size_t(0)
+        rc = lib().sf_ws_sync(self.h, ctypes.byref(found))
+        if rc not in (0, -2):
+            _check(rc)
+        return rc, found.value
+
+    def fetch(self, n):
+        out = np.zeros(n, SIGNAL_PIXEL)
+        _check(lib().sf_ws_fetch(self.h, _ptr(out), n))
+        return out
+
+    def set_timing(self, on):
+        lib().sf_ws_set_timing(self.h, int(bool(on)))
+
+    def kernel_ms(self):
+        """(threshold_ms, order_ms) of the last threshold, timing on."""
+        a, b = ctypes.c_double(0), ctypes.c_double(0)
+        _check(lib().sf_ws_kernel_ms(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def fallbacks(self):
+        return int(lib().sf_ws_fallbacks(self.h))
+
+    def device_signal(self):
+        return lib().sf_ws_device_signal(self.h)
+
+    def device_mask(self):
+        wpr = ctypes.c_uint32(0)
+        p = lib().sf_ws_device_mask(self.h, ctypes.byref(wpr))
+        return p, wpr.value
+
+
+def synth_desc(kind="4m", **overrides):
+    """A frame description: "4m" / "16m" EIGER2 geometry, or (w, h) plain."""
+    d = SynthDesc()
+    if kind == "4m":
+        lib().sf_synth_eiger(ctypes.byref(d), 2, 4)
+    elif kind == "16m":
+        lib().sf_synth_eiger(ctypes.byref(d), 4, 8)
+    elif kind == "9m":
+        lib().sf_synth_default(ctypes.byref(d), 3110, 3269)
+        d.module_w, d.gap_w, d.module_h, d.gap_h = 1030, 10, 514, 37
+    else:
+        w, h = kind
+        lib().sf_synth_default(ctypes.byref(d), w, h)
+    for k, v in overrides.items():
+        setattr(d, k, v)
+    return d
+
+
+def synth_frame(desc, frame=0, out=None):
+    dt = np.uint16 if desc.bits == 16 else np.uint32
+    if out is None:
+        out = np.empty((desc.height, desc.width), dt)
+    rc = lib().sf_synth_frame(ctypes.byref(desc), int(frame), _ptr(out))
+    if rc != 0:
+        raise ValueError("bad synthetic frame description")
+    return out

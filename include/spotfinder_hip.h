@@ -1,0 +1,171 @@
+/*
+ * spotfinder_hip: the C ABI of the MI355X (gfx950) backend for the reference's
+ * spot-finding hot path -- the extended dispersion threshold, strong-pixel
+ * connected-component labelling and per-spot reductions of src/spots.
+ *
+ * Plain C: pointers, sizes and POD records, integer status codes, no
+ * exceptions and no HIP or C++ types.  The C++ gpu:: interface of the
+ * reference (src/spots/dext_gpu.hh) is implemented on top of this in
+ * dext_gpu.hh / dext_gpu_hip.cc; a binding from any other language calls these
+ * functions directly.
+ *
+ * Status codes: 0 success; -1 bad argument; -2 more signal pixels than the
+ * device list holds (nothing is returned, as dext_cuda.cu:821-822); -3 a
+ * device or runtime failure, with the runtime's message in sf_last_error();
+ * -4 out of memory; -5 unsupported request.  sf_last_error() describes the
+ * last failure on the calling thread.
+ */
+#ifndef SPOTFINDER_HIP_H
+#define SPOTFINDER_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ------------------------------------------------------------------------
+ * Records.  Layouts are the reference's, byte for byte.
+ * ------------------------------------------------------------------------ */
+
+/* SignalPixel, src/spots/signal_pixel.hh:20-28. */
+typedef struct sf_signal_pixel {
+  uint32_t index;      /* row-major position in the frame */
+  uint32_t value;      /* the raw pixel */
+  float background;    /* the masked local mean at that pixel */
+  uint16_t population; /* pixels that mean was averaged over */
+  uint16_t reserved;
+} sf_signal_pixel;
+
+/* dials_spots::Pixel, src/spots/dials_spots.hh:50-54. */
+typedef struct sf_pixel {
+  int32_t frame;
+  uint32_t index;
+  uint32_t value;
+} sf_pixel;
+
+/* dials_spots::Spot, src/spots/dials_spots.hh:57-79. */
+typedef struct sf_spot {
+  int32_t bbox[6]; /* x0, x1, y0, y1, z0, z1; upper bounds exclusive */
+  double position[3];
+  double variance[3];
+  double intensity;
+  double intensity_variance;
+  uint32_t n_signal;
+  uint64_t first; /* where its pixels start in the pixel list */
+} sf_spot;
+
+/* dials_spots::Counts, src/spots/dials_spots.hh:99-106. */
+typedef struct sf_counts {
+  uint64_t signal_pixels, groups, too_small, too_large, separated, accepted;
+} sf_counts;
+
+/* One four-connected component of one frame, reduced on the device.  New:
+ * the reference has no device labelling.  All sums are exact integers; the
+ * x and y moments are taken at doubled pixel centres (2x+1, 2y+1) so that
+ * they stay integers, and the 128-bit sums are split into lo/hi words. */
+typedef struct sf_component {
+  uint32_t first;     /* lowest flat index in the component */
+  uint32_t n;         /* pixels */
+  uint32_t x0, x1;    /* bbox columns, x1 exclusive */
+  uint32_t y0, y1;    /* bbox rows, y1 exclusive */
+  uint32_t peak_value;
+  uint32_t peak_index; /* lowest index among the brightest pixels */
+  uint64_t sum;        /* sum of values */
+  uint64_t sum_sq[2];  /* sum of value^2, lo/hi */
+  uint64_t sum_x[2];   /* sum of value*(2x+1), lo/hi */
+  uint64_t sum_y[2];   /* sum of value*(2y+1), lo/hi */
+  uint64_t sum_xx[2];  /* sum of value*(2x+1)^2, lo/hi */
+  uint64_t sum_yy[2];  /* sum of value*(2y+1)^2, lo/hi */
+} sf_component;
+
+/* ------------------------------------------------------------------------
+ * Device and host memory: gpu::available, unavailable_reason, memory_free,
+ * host_alloc, host_free (src/spots/dext_gpu.hh:137-154), backend (:29).
+ * ------------------------------------------------------------------------ */
+
+const char *sf_backend(void);              /* "HIP" */
+int sf_available(void);                    /* 1 if a device is usable */
+const char *sf_unavailable_reason(void);   /* "" when available */
+size_t sf_memory_free(void);               /* device bytes free, 0 on error */
+void *sf_host_alloc(size_t bytes);         /* pinned; NULL on failure */
+void sf_host_free(void *pointer);
+const char *sf_last_error(void);           /* this thread's last failure */
+
+/* ------------------------------------------------------------------------
+ * Kernel choice: replaces gpu::stage0_window/stage2_window/fused
+ * (src/spots/dext_gpu.hh:50-82).  Both compute the same thing.
+ *   SF_KERNEL_FUSED   one fused kernel per frame (default)
+ *   SF_KERNEL_DIRECT  three direct-window kernels, the cross-check
+ * ------------------------------------------------------------------------ */
+
+enum { SF_KERNEL_FUSED = 0, SF_KERNEL_DIRECT = 1 };
+void sf_set_kernel(int kernel);
+int sf_get_kernel(void);
+
+/* Per-stage times of this thread's last profiled find, gpu::StageTimes
+ * (src/spots/dext_gpu.hh:89-112), in milliseconds. */
+typedef struct sf_stage_times {
+  double upload, stage0, stage1, stage2, fused, copy, sort;
+} sf_stage_times;
+void sf_profile_stages(int on);
+int sf_profiling_stages(void);
+sf_stage_times sf_last_stage_times(void);
+
+/* ------------------------------------------------------------------------
+ * gpu::find<T> (src/spots/dext_gpu.hh:171-180; dext_cuda.cu:710-869), split
+ * into run and fetch so a caller sizes its list once it knows the count.
+ *
+ * sf_find_run thresholds one host frame (pixel_bits 16 or 32) on this
+ * thread's own stream and workspace, and leaves the signal pixels on the
+ * device, ascending by index.  *found is the count.  -1 for a null frame, a
+ * zero size or height*width above INT32_MAX; -2 if the count exceeds the
+ * device list (frame bytes / 16 entries, the reference's bound).
+ *
+ * sf_find_fetch copies the first n of them to `out`.  No work is in flight
+ * when either returns.
+ * ------------------------------------------------------------------------ */
+
+int sf_find_run(int pixel_bits, const void *frame, size_t height, size_t width,
+                size_t *found);
+int sf_find_fetch(sf_signal_pixel *out, size_t n);
+
+/* ------------------------------------------------------------------------
+ * Device-resident frames.  New surface, for callers that keep frames in HBM
+ * (the batched and serving paths, and the benchmark).
+ *
+ * A workspace holds the device buffers for frames up to max_pixels and one
+ * stream.  sf_ws_threshold enqueues the threshold of one device frame and
+ * the ordering of its signal pixels on that stream and returns at once;
+ * sf_ws_sync waits and reports the count; results stay on the device until
+ * the next call.  `stream` (a hipStream_t, or NULL for the workspace's own)
+ * is borrowed, never destroyed.
+ * ------------------------------------------------------------------------ */
+
+typedef struct sf_workspace sf_workspace;
+
+int sf_ws_create(size_t max_pixels, void *stream, sf_workspace **out);
+void sf_ws_destroy(sf_workspace *ws);
+void *sf_ws_stream(sf_workspace *ws);
+int sf_ws_threshold(sf_workspace *ws, int pixel_bits, const void *dev_frame,
+                    uint32_t height, uint32_t width, int kernel);
+int sf_ws_sync(sf_workspace *ws, size_t *found);
+int sf_ws_fetch(sf_workspace *ws, sf_signal_pixel *out, size_t n);
+/* With timing on, the last sf_ws_threshold's kernel times in milliseconds
+ * from HIP events on the workspace's stream: the threshold kernel(s) alone,
+ * and the ordering kernels that follow. */
+void sf_ws_set_timing(sf_workspace *ws, int on);
+int sf_ws_kernel_ms(sf_workspace *ws, double *threshold_ms, double *order_ms);
+/* Frames this workspace's fused kernel handed back to the direct kernels
+ * because a valid pixel was beyond its exact fast range. */
+uint64_t sf_ws_fallbacks(sf_workspace *ws);
+/* Device pointers of the last frame's results. */
+const sf_signal_pixel *sf_ws_device_signal(sf_workspace *ws);
+const uint32_t *sf_ws_device_mask(sf_workspace *ws, uint32_t *words_per_row);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,209 @@
+"""ctypes bindings for the oracle (liboracle.so) and for the reference's own
+CPU code compiled into oracle/_ref/libspots_ref.so.
+
+Test infrastructure only: tests/, __graft_entry__.smoke() and bench.py's CPU
+baseline import this module; the product never does.
+
+Reference interfaces these mirror (paths in the reference repository):
+  dext<T>                     src/spots/dext.hh:96-107
+  signal_order::by_index      src/spots/signal_order.hh:62-66
+  dials_spots::Labeller       src/spots/dials_spots.hh:108-158
+"""
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+SIGNAL_PIXEL = np.dtype(
+    [("index", "<u4"), ("value", "<u4"), ("background", "<f4"),
+     ("population", "<u2"), ("reserved", "<u2")])
+PIXEL = np.dtype([("frame", "<i4"), ("index", "<u4"), ("value", "<u4")])
+SPOT = np.dtype(
+    [("bbox", "<i4", (6,)), ("position", "<f8", (3,)), ("variance", "<f8", (3,)),
+     ("intensity", "<f8"), ("intensity_variance", "<f8"), ("n_signal", "<u4"),
+     ("first", "<u8")], align=True)
+COUNTS_FIELDS = ("signal_pixels", "groups", "too_small", "too_large",
+                 "separated", "accepted")
+assert SIGNAL_PIXEL.itemsize == 16 and PIXEL.itemsize == 12
+assert SPOT.itemsize == 104
+
+
+class _Counts(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in COUNTS_FIELDS]
+
+
+def _p(a):
+    return a.ctypes.data_as(ctypes.c_void_p)
+
+
+class _Lib:
+    """Shared loader for the two libraries, which expose the same surface
+    under the prefixes oracle_ and ref_."""
+
+    def __init__(self, path, prefix):
+        self.path = path
+        self.lib = ctypes.CDLL(path)
+        self.prefix = prefix
+        L, P = self.lib, prefix
+        vp, sz = ctypes.c_void_p, ctypes.c_size_t
+        for t in ("u16", "u32"):
+            f = getattr(L, f"{P}dext_{t}")
+            f.argtypes = [vp, sz, sz, vp, sz, ctypes.POINTER(sz)]
+            f.restype = ctypes.c_int
+        getattr(L, f"{P}sort_by_index").argtypes = [vp, sz]
+        getattr(L, f"{P}sort_by_index").restype = None
+        getattr(L, f"{P}labeller_new").argtypes = [
+            sz, sz, sz, sz, ctypes.c_double, ctypes.c_int, ctypes.c_char_p, sz]
+        getattr(L, f"{P}labeller_new").restype = vp
+        getattr(L, f"{P}labeller_free").argtypes = [vp]
+        getattr(L, f"{P}labeller_add").argtypes = [
+            vp, ctypes.c_int64, vp, sz, ctypes.c_char_p, sz]
+        getattr(L, f"{P}labeller_add").restype = ctypes.c_int
+        getattr(L, f"{P}labeller_finish").argtypes = [vp]
+        getattr(L, f"{P}labeller_spot_count").argtypes = [vp]
+        getattr(L, f"{P}labeller_spot_count").restype = sz
+        getattr(L, f"{P}labeller_pixel_count").argtypes = [vp]
+        getattr(L, f"{P}labeller_pixel_count").restype = sz
+        getattr(L, f"{P}labeller_live").argtypes = [vp]
+        getattr(L, f"{P}labeller_live").restype = sz
+        f = getattr(L, f"{P}dext_u32_threads")
+        f.argtypes = [vp, sz, sz, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        f.restype = ctypes.c_uint64
+        if prefix == "oracle_":
+            L.oracle_labeller_spots.argtypes = [vp]
+            L.oracle_labeller_spots.restype = vp
+            L.oracle_labeller_pixels.argtypes = [vp]
+            L.oracle_labeller_pixels.restype = vp
+            L.oracle_labeller_counts.argtypes = [vp]
+            L.oracle_labeller_counts.restype = _Counts
+        else:
+            L.ref_labeller_spots.argtypes = [vp, vp]
+            L.ref_labeller_pixels.argtypes = [vp, vp]
+            L.ref_labeller_counts.argtypes = [vp, ctypes.POINTER(_Counts)]
+            L.ref_sizeof_spot.restype = sz
+
+    def dext(self, image, capacity=None):
+        """(status, SignalPixel array) for a 2-D uint16 or uint32 frame."""
+        image = np.ascontiguousarray(image)
+        if image.dtype == np.uint16:
+            f = getattr(self.lib, f"{self.prefix}dext_u16")
+        elif image.dtype == np.uint32:
+            f = getattr(self.lib, f"{self.prefix}dext_u32")
+        else:
+            raise TypeError(f"no threshold for {image.dtype}")
+        h, w = image.shape
+        if capacity is None:
+            capacity = h * w
+        out = np.zeros(max(capacity, 1), SIGNAL_PIXEL)
+        found = ctypes.c_size_t(0)
+        rc = f(_p(image), h, w, _p(out), capacity, ctypes.byref(found))
+        return rc, out[: min(found.value, capacity)].copy()
+
+    def sort_by_index(self, pixels):
+        pixels = np.ascontiguousarray(pixels, SIGNAL_PIXEL).copy()
+        getattr(self.lib, f"{self.prefix}sort_by_index")(_p(pixels), len(pixels))
+        return pixels
+
+    def threads_u32(self, frames, threads):
+        frames = np.ascontiguousarray(frames, np.uint32)
+        n, h, w = frames.shape
+        secs = ctypes.c_double(0)
+        sig = getattr(self.lib, f"{self.prefix}dext_u32_threads")(
+            _p(frames), n, h, w, threads, ctypes.byref(secs))
+        return int(sig), secs.value
+
+    def labeller(self, height, width, min_spot_size=3, max_spot_size=1000,
+                 max_separation=2.0, two_d=False):
+        return Labeller(self, height, width, min_spot_size, max_spot_size,
+                        max_separation, two_d)
+
+
+class LabellerError(RuntimeError):
+    pass
+
+
+class Labeller:
+    def __init__(self, lib, h, w, mn, mx, sep, two_d):
+        self.L = lib
+        err = ctypes.create_string_buffer(512)
+        self.h = getattr(lib.lib, f"{lib.prefix}labeller_new")(
+            h, w, mn, mx, sep, int(bool(two_d)), err, 512)
+        if not self.h:
+            raise LabellerError(err.value.decode())
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            getattr(self.L.lib, f"{self.L.prefix}labeller_free")(self.h)
+            self.h = None
+
+    def add(self, frame, signal):
+        signal = np.ascontiguousarray(signal, SIGNAL_PIXEL)
+        err = ctypes.create_string_buffer(512)
+        rc = getattr(self.L.lib, f"{self.L.prefix}labeller_add")(
+            self.h, int(frame), _p(signal), len(signal), err, 512)
+        if rc != 0:
+            raise LabellerError(err.value.decode())
+
+    def finish(self):
+        getattr(self.L.lib, f"{self.L.prefix}labeller_finish")(self.h)
+
+    def live(self):
+        return getattr(self.L.lib, f"{self.L.prefix}labeller_live")(self.h)
+
+    def spots(self):
+        n = getattr(self.L.lib, f"{self.L.prefix}labeller_spot_count")(self.h)
+        out = np.zeros(n, SPOT)
+        if n:
+            if self.L.prefix == "oracle_":
+                ptr = self.L.lib.oracle_labeller_spots(self.h)
+                ctypes.memmove(_p(out), ptr, n * SPOT.itemsize)
+            else:
+                self.L.lib.ref_labeller_spots(self.h, _p(out))
+        return out
+
+    def pixels(self):
+        n = getattr(self.L.lib, f"{self.L.prefix}labeller_pixel_count")(self.h)
+        out = np.zeros(n, PIXEL)
+        if n:
+            if self.L.prefix == "oracle_":
+                ptr = self.L.lib.oracle_labeller_pixels(self.h)
+                ctypes.memmove(_p(out), ptr, n * PIXEL.itemsize)
+            else:
+                self.L.lib.ref_labeller_pixels(self.h, _p(out))
+        return out
+
+    def counts(self):
+        if self.L.prefix == "oracle_":
+            c = self.L.lib.oracle_labeller_counts(self.h)
+        else:
+            c = _Counts()
+            self.L.lib.ref_labeller_counts(self.h, ctypes.byref(c))
+        return {f: int(getattr(c, f)) for f in COUNTS_FIELDS}
+
+
+_oracle = None
+_ref = None
+
+
+def oracle():
+    """The oracle's own library, built by `make -C oracle`."""
+    global _oracle
+    if _oracle is None:
+        path = os.path.join(HERE, "liboracle.so")
+        if not os.path.exists(path):
+            raise FileNotFoundError(f"{path} is not built; run make -C oracle")
+        _oracle = _Lib(path, "oracle_")
+    return _oracle
+
+
+def reference():
+    """The reference's CPU code (oracle/_ref), or None where it was not built."""
+    global _ref
+    if _ref is None:
+        path = os.path.join(HERE, "_ref", "libspots_ref.so")
+        if not os.path.exists(path):
+            return None
+        _ref = _Lib(path, "ref_")
+    return _ref

@@ -1,0 +1,80 @@
+"""The C ABI without a device: the library loads, exports every function the
+public headers declare, and refuses bad arguments before touching a device."""
+import ctypes
+import glob
+import os
+import re
+
+import numpy as np
+
+from conftest import ROOT
+
+
+def declared_functions():
+    names = set()
+    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+        text = open(h).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"\b(sf_[a-z0-9_]+)\s*\(", text):
+            names.add(m.group(1))
+    return names
+
+
+def test_headers_declare_the_surface():
+    names = declared_functions()
+    for must in ("sf_find_run", "sf_find_fetch", "sf_available", "sf_backend",
+                 "sf_ws_threshold", "sf_synth_frame"):
+        assert must in names
+
+
+def test_library_exports_every_declared_function(sf):
+    lib = ctypes.CDLL(sf.LIBRARY)
+    missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_backend_and_reason(sf):
+    assert sf.backend() == "HIP"
+    if not sf.available():
+        assert sf.unavailable_reason() != ""
+    else:
+        assert sf.unavailable_reason() == ""
+
+
+def test_bad_arguments_are_refused_without_a_device(sf):
+    L = sf.lib()
+    found = ctypes.c_size_t(7)
+    frame = np.zeros((4, 4), np.uint32)
+    p = frame.ctypes.data_as(ctypes.c_void_p)
+    assert L.sf_find_run(32, None, 4, 4, ctypes.byref(found)) == -1
+    assert found.value == 0
+    assert L.sf_find_run(32, p, 0, 4, ctypes.byref(found)) == -1
+    assert L.sf_find_run(32, p, 4, 0, ctypes.byref(found)) == -1
+    assert L.sf_find_run(32, p, 1 << 20, 1 << 12, ctypes.byref(found)) == -1
+    assert L.sf_find_run(8, p, 4, 4, ctypes.byref(found)) == -1
+    assert sf.find(np.zeros((0, 5), np.uint16))[0] == -1
+
+
+def test_record_layouts_match_the_reference():
+    from oracle import oracle as O
+    import spotfinder_hip as sfm
+    assert sfm.SIGNAL_PIXEL == O.SIGNAL_PIXEL and sfm.SIGNAL_PIXEL.itemsize == 16
+    assert sfm.PIXEL.itemsize == 12 and sfm.SPOT.itemsize == 104
+    ref = O.reference()
+    if ref is not None:
+        assert ref.lib.ref_sizeof_spot() == 104
+
+
+def test_synthetic_frames_are_deterministic(sf):
+    d = sf.synth_desc((300, 200), seed=5, spots_per_frame=20)
+    a = sf.synth_frame(d, 3)
+    d.threads = 1
+    b = sf.synth_frame(d, 3)
+    assert a.tobytes() == b.tobytes()
+    assert sf.synth_frame(d, 4).tobytes() != a.tobytes()
+    g = sf.synth_frame(sf.synth_desc("4m"), 0)
+    assert g.shape == (2162, 2068)
+    # module gaps: 12 columns after each 1028, 38 rows after each 512
+    assert (g[:, 1028:1040] == 0xFFFFFFFF).all()
+    assert (g[512:550, :] == 0xFFFFFFFF).all()
+    assert ((g == 65535).sum() > 100)

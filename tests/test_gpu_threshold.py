@@ -1,0 +1,132 @@
+"""The HIP threshold against the reference, through the C ABI, on the device.
+
+Bit-exact on every field of every SignalPixel (index, value, background float
+bits, population), as the reference's own device test demands
+(tests/spots/test_dext_gpu.cc:73-110), for both kernels: the fused
+single-pass kernel and the three direct-window kernels.  Inputs are the golden
+size/edge matrix and EIGER2-shaped synthetic frames at full size (4M, 9M,
+16M, 512x512 uint16, and a frame whose values exceed the fused kernel's fast
+range so that the exact fallback runs); expected outputs come from the
+reference's dext.cc compiled as is.
+"""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+KERNELS = ["fused", "direct"]
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(params=KERNELS)
+def kernel(request, device):
+    device.set_kernel(request.param)
+    yield request.param
+    device.set_kernel("fused")
+
+
+def test_matrix_bit_exact(device, kernel, matrix):
+    for name, (frame, expected) in matrix.items():
+        say(kernel, name, frame.shape, frame.dtype)
+        rc, got = device.find(frame)
+        assert rc == 0, name
+        assert len(got) == len(expected), (name, len(got), len(expected))
+        assert got.tobytes() == expected.tobytes(), name
+
+
+def test_synthetic_frames_bit_exact(device, kernel, synth_cases):
+    for name, case in synth_cases.items():
+        kind = case["kind"] if isinstance(case["kind"], str) else tuple(case["kind"])
+        frame = device.synth_frame(device.synth_desc(kind, **case["overrides"]),
+                                   case["frame"])
+        assert hashlib.sha256(frame.tobytes()).hexdigest() == case["frame_sha256"]
+        say(kernel, name, frame.shape, frame.dtype)
+        rc, got = device.find(frame)
+        assert rc == case["status"], name
+        assert len(got) == case["count"], (name, len(got), case["count"])
+        assert hashlib.sha256(got.tobytes()).hexdigest() == case["signal_sha256"], name
+
+
+def test_edge_shapes_against_oracle(device, kernel, oracle_lib):
+    rng = np.random.default_rng(99)
+    shapes = [(1, 1), (1, 700), (700, 1), (2, 2), (11, 233), (233, 11),
+              (21, 232), (64, 255), (64, 256), (64, 257), (300, 465), (3, 1000)]
+    for (h, w) in shapes:
+        for dt in (np.uint16, np.uint32):
+            top = np.iinfo(dt).max
+            f = rng.poisson(3.0, (h, w)).astype(np.int64)
+            s = rng.uniform(size=(h, w)) < 0.03
+            f[s] += rng.integers(50, 5000, s.sum())
+            f = np.minimum(f, top - 2)
+            f[rng.uniform(size=(h, w)) < 0.02] = top - 1
+            f = f.astype(dt)
+            say(kernel, "edge", h, w, dt)
+            rc, got = device.find(f)
+            _, want = oracle_lib.dext(f)
+            assert rc == 0 and got.tobytes() == want.tobytes(), (h, w, dt)
+
+
+def test_all_masked_and_empty(device, kernel):
+    for dt in (np.uint16, np.uint32):
+        top = np.iinfo(dt).max
+        f = np.full((64, 64), top - 1, dt)
+        f[::7, ::3] = top
+        rc, got = device.find(f)
+        assert rc == 0 and len(got) == 0
+        rc, got = device.find(np.zeros((128, 300), dt))
+        assert rc == 0 and len(got) == 0
+
+
+def test_over_capacity_returns_minus_two(device, kernel):
+    # Alternating columns of 0 and 10000: half the pixels are signal, above
+    # the list's one entry per four pixels (eight at 16 bits).
+    for dt in (np.uint32, np.uint16):
+        f = np.zeros((256, 512), dt)
+        f[:, ::2] = 10000
+        say(kernel, "over capacity", dt)
+        rc, got = device.find(f)
+        assert rc == -2 and len(got) == 0
+    # and the workspace recovers for the next frame
+    f = np.zeros((64, 64), np.uint32)
+    f[30, 30] = 500
+    rc, got = device.find(f)
+    assert rc == 0 and len(got) == 1
+
+
+def test_bad_arguments(device):
+    assert device.find(np.zeros((0, 3), np.uint32))[0] == -1
+
+
+def test_device_workspace_matches_find(device, synth_cases):
+    import torch
+    case = synth_cases["eiger4m_f0"]
+    frame = device.synth_frame(device.synth_desc("4m", **case["overrides"]),
+                               case["frame"])
+    d = torch.from_numpy(frame.view(np.int32)).to("cuda")
+    ws = device.Workspace()
+    for kernel in KERNELS:
+        say("workspace", kernel)
+        ws.threshold(d.data_ptr(), 32, frame.shape[0], frame.shape[1], kernel)
+        rc, n = ws.sync()
+        assert rc == 0 and n == case["count"]
+        got = ws.fetch(n)
+        assert hashlib.sha256(got.tobytes()).hexdigest() == case["signal_sha256"]
+    ws.close()
+
+
+def test_stage_times(device):
+    device.profile_stages(True)
+    try:
+        f = device.synth_frame(device.synth_desc("4m"), 0)
+        rc, _ = device.find(f)
+        t = device.last_stage_times()
+        assert rc == 0 and t["upload"] > 0 and t["fused"] > 0
+    finally:
+        device.profile_stages(False)
