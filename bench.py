@@ -65,7 +65,6 @@ def main():
               f"would shard", file=sys.stderr)
         sys.exit(2)
 
-    import torch
     import spotfinder_hip as sf
     if not sf.available():
         raise SystemExit("no HIP device: " + sf.unavailable_reason())
@@ -74,8 +73,8 @@ def main():
     h, w = desc.height, desc.width
     log(f"generating {args.frames} frames of {w} x {h} uint32")
     frames_host = [sf.synth_frame(desc, z) for z in range(args.frames)]
-    dev = [torch.from_numpy(f.view(np.int32)).to("cuda") for f in frames_host]
-    torch.cuda.synchronize()
+    dev = [sf.DeviceArray.from_numpy(f) for f in frames_host]
+    sf.synchronize()
 
     ws = sf.Workspace()
     ws.set_timing(True)
@@ -84,7 +83,7 @@ def main():
     def step(record):
         kms = []
         for i, d in enumerate(dev):
-            ws.threshold(d.data_ptr(), 32, h, w, args.kernel)
+            ws.threshold(d.ptr, 32, h, w, args.kernel)
             rc, n = ws.sync()
             if rc != 0:
                 raise RuntimeError(f"frame {i}: status {rc}")
@@ -96,11 +95,11 @@ def main():
     for _ in range(args.warmup):
         step(False)
     ws.set_timing(False)
-    torch.cuda.synchronize()
+    sf.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
         step(False)
-    torch.cuda.synchronize()
+    sf.synchronize()
     elapsed = time.perf_counter() - t0
 
     # Kernel durations, measured live with HIP events on the kernel's stream.

@@ -1,0 +1,268 @@
+// Strong-pixel connected components of one frame, and their exact moments,
+// on the device.
+//
+// The reference groups on the host (dials_spots::Labeller::add,
+// src/spots/dials_spots.cc:57-145): pixels are unioned with their left
+// neighbour (same row) and the one above, toward the lower position, and a
+// component's rows are emitted in order of its first pixel.  Here the same
+// forest is built in the rank space of the frame's ordered signal list:
+//   * the left neighbour of rank i, when set, is rank i - 1; the one above is
+//     found from the strong-pixel bitmask and its word prefix;
+//   * a lock-free union-find hooks the larger root under the smaller, so the
+//     root of every component is its lowest rank, i.e. its first pixel;
+//   * roots are numbered by a popcount scan of a root bitmask, which numbers
+//     components in order of first appearance -- the reference's order;
+//   * every pixel adds its exact integer moments to its component with
+//     atomics (128-bit where a product can exceed 64 bits).
+// Links to the previous frame (same index set in both frames) are listed for
+// the host's three-dimensional stitch (dials_spots.cc:128-137).
+#include "sf_device.hh"
+#include "sf_internal.hh"
+
+namespace {
+
+__device__ __forceinline__ uint32_t load_relaxed(const uint32_t *p) {
+  return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void store_relaxed(uint32_t *p, uint32_t v) {
+  __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+// The root of i, shortening the path as it goes.  Only non-roots are written,
+// and only with one of their ancestors, so a concurrent hook of a root is
+// never undone.
+__device__ uint32_t find(uint32_t *parent, uint32_t i) {
+  uint32_t cur = load_relaxed(&parent[i]);
+  if (cur == i)
+    return i;
+  uint32_t prev = i;
+  for (;;) {
+    const uint32_t next = load_relaxed(&parent[cur]);
+    if (next == cur)
+      return cur;
+    store_relaxed(&parent[prev], next);
+    prev = cur;
+    cur = next;
+  }
+}
+
+__device__ void unite(uint32_t *parent, uint32_t a, uint32_t b) {
+  uint32_t ra = find(parent, a), rb = find(parent, b);
+  while (ra != rb) {
+    if (ra < rb) {
+      const uint32_t t = ra;
+      ra = rb;
+      rb = t;
+    }
+    const uint32_t old = atomicCAS(&parent[ra], ra, rb);
+    if (old == ra)
+      return;
+    ra = find(parent, old);
+    rb = find(parent, rb);
+  }
+}
+
+__device__ __forceinline__ bool bit_set(const uint32_t *mask, uint32_t wpr,
+                                        uint32_t x, uint32_t y) {
+  return (mask[static_cast<size_t>(y) * wpr + x / 32] >> (x & 31u)) & 1u;
+}
+
+__device__ __forceinline__ uint32_t rank_of(const uint32_t *mask,
+                                            const uint32_t *prefix, uint32_t wpr,
+                                            uint32_t x, uint32_t y) {
+  const size_t w = static_cast<size_t>(y) * wpr + x / 32;
+  return prefix[w] + static_cast<uint32_t>(__popc(mask[w] & ((1u << (x & 31u)) - 1u)));
+}
+
+__device__ __forceinline__ uint32_t live_count(const uint32_t *counter,
+                                               uint32_t capacity) {
+  const uint32_t n = *counter;
+  return n > capacity ? 0u : n; // an overfull frame is refused, not labelled
+}
+
+// parent[i] = i - 1 when the left neighbour is set, else i.
+__global__ __launch_bounds__(256) void link_left(
+    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
+    uint32_t capacity, uint32_t width, uint32_t *__restrict__ parent) {
+  const uint32_t n = live_count(counter, capacity);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t idx = list[i].index;
+    const bool left = i > 0 && idx % width != 0 && list[i - 1].index == idx - 1;
+    parent[i] = left ? i - 1 : i;
+  }
+}
+
+// Union with the pixel above.
+__global__ __launch_bounds__(256) void link_up(
+    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
+    uint32_t capacity, uint32_t width, const uint32_t *__restrict__ mask,
+    const uint32_t *__restrict__ prefix, uint32_t wpr, uint32_t *parent) {
+  const uint32_t n = live_count(counter, capacity);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t idx = list[i].index;
+    const uint32_t y = idx / width, x = idx - y * width;
+    if (y > 0 && bit_set(mask, wpr, x, y - 1))
+      unite(parent, i, rank_of(mask, prefix, wpr, x, y - 1));
+  }
+}
+
+// Flatten every tree to its root; mark the roots in a bitmask of ranks.
+// A wave covers 64 consecutive ranks from a multiple of 64, so its ballot is
+// two whole words.
+__global__ __launch_bounds__(256) void flatten(
+    const uint32_t *__restrict__ counter, uint32_t capacity, uint32_t *parent,
+    uint32_t *__restrict__ roots) {
+  const uint32_t n = live_count(counter, capacity);
+  const uint32_t span = (n + 63u) & ~63u;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < span;
+       i += gridDim.x * blockDim.x) {
+    bool root = false;
+    if (i < n) {
+      uint32_t r = parent[i];
+      while (parent[r] != r)
+        r = parent[r];
+      parent[i] = r;
+      root = r == i;
+    }
+    const uint64_t b = __ballot(root);
+    const uint32_t lane = sfd::lane_id();
+    if (lane == 0)
+      roots[i / 32] = static_cast<uint32_t>(b);
+    else if (lane == 32)
+      roots[i / 32] = static_cast<uint32_t>(b >> 32);
+  }
+}
+
+__device__ __forceinline__ uint32_t component_of(const uint32_t *roots,
+                                                 const uint32_t *root_prefix,
+                                                 uint32_t r) {
+  return root_prefix[r / 32] +
+         static_cast<uint32_t>(__popc(roots[r / 32] & ((1u << (r & 31u)) - 1u)));
+}
+
+// Component of every pixel; each root opens its component's record.
+__global__ __launch_bounds__(256) void number_components(
+    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
+    uint32_t capacity, const uint32_t *__restrict__ parent,
+    const uint32_t *__restrict__ roots, const uint32_t *__restrict__ root_prefix,
+    uint32_t *__restrict__ comp_of, sf_component *__restrict__ comps,
+    uint32_t *__restrict__ ncomp) {
+  const uint32_t n = live_count(counter, capacity);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t r = parent[i];
+    const uint32_t c = component_of(roots, root_prefix, r);
+    comp_of[i] = c;
+    if (r == i) {
+      sf_component z{};
+      z.first = list[i].index;
+      z.x0 = z.y0 = 0xFFFFFFFFu;
+      comps[c] = z;
+      atomicAdd(ncomp, 1u);
+    }
+  }
+}
+
+__device__ __forceinline__ void add128(uint64_t *pair, uint64_t lo, uint64_t hi) {
+  const unsigned long long old =
+      atomicAdd(reinterpret_cast<unsigned long long *>(&pair[0]),
+                static_cast<unsigned long long>(lo));
+  const uint64_t carry = (static_cast<uint64_t>(old) + lo) < lo ? 1u : 0u;
+  if (hi + carry)
+    atomicAdd(reinterpret_cast<unsigned long long *>(&pair[1]),
+              static_cast<unsigned long long>(hi + carry));
+}
+
+// Each pixel's exact moments into its component.
+__global__ __launch_bounds__(256) void accumulate(
+    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
+    uint32_t capacity, uint32_t width, const uint32_t *__restrict__ comp_of,
+    sf_component *comps) {
+  const uint32_t n = live_count(counter, capacity);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const sf_signal_pixel p = list[i];
+    const uint32_t y = p.index / width, x = p.index - y * width;
+    sf_component *c = &comps[comp_of[i]];
+    const uint64_t v = p.value;
+    const uint64_t ax = 2ull * x + 1ull, ay = 2ull * y + 1ull;
+    atomicAdd(&c->n, 1u);
+    atomicMin(&c->x0, x);
+    atomicMax(&c->x1, x + 1);
+    atomicMin(&c->y0, y);
+    atomicMax(&c->y1, y + 1);
+    atomicMax(reinterpret_cast<unsigned long long *>(&c->peak),
+              static_cast<unsigned long long>((v << 32) | (0xFFFFFFFFu - p.index)));
+    atomicAdd(reinterpret_cast<unsigned long long *>(&c->sum),
+              static_cast<unsigned long long>(v));
+    add128(c->sum_sq, v * v, 0);
+    add128(c->sum_x, v * ax, 0);
+    add128(c->sum_y, v * ay, 0);
+    const uint64_t xx = ax * ax, yy = ay * ay;
+    add128(c->sum_xx, v * xx, __umul64hi(v, xx));
+    add128(c->sum_yy, v * yy, __umul64hi(v, yy));
+  }
+}
+
+// Pairs (component here, component on the previous frame) for every pixel
+// set in both frames; a pixel repeating its left neighbour's pair is skipped.
+__global__ __launch_bounds__(256) void link_frames(
+    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
+    uint32_t capacity, uint32_t width, const uint32_t *__restrict__ comp_of,
+    const uint32_t *__restrict__ prev_mask, const uint32_t *__restrict__ prev_prefix,
+    const uint32_t *__restrict__ prev_comp_of, uint32_t wpr,
+    uint32_t *__restrict__ links, uint32_t *__restrict__ nlinks,
+    uint32_t link_capacity) {
+  const uint32_t n = live_count(counter, capacity);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t idx = list[i].index;
+    const uint32_t y = idx / width, x = idx - y * width;
+    if (!bit_set(prev_mask, wpr, x, y))
+      continue;
+    const uint32_t here = comp_of[i];
+    const uint32_t there = prev_comp_of[rank_of(prev_mask, prev_prefix, wpr, x, y)];
+    if (x > 0 && i > 0 && list[i - 1].index == idx - 1 && comp_of[i - 1] == here &&
+        bit_set(prev_mask, wpr, x - 1, y) &&
+        prev_comp_of[rank_of(prev_mask, prev_prefix, wpr, x - 1, y)] == there)
+      continue;
+    const uint32_t at = atomicAdd(nlinks, 1u);
+    if (at < link_capacity) {
+      links[2 * at] = here;
+      links[2 * at + 1] = there;
+    }
+  }
+}
+
+} // namespace
+
+namespace sfi {
+
+void launch_label(const LabelArgs &a, hipStream_t stream) {
+  const uint32_t blocks = 512;
+  launch_fill(a.ncomp, 1, 0u, stream);
+  link_left<<<blocks, 256, 0, stream>>>(a.list, a.counter, a.capacity, a.width,
+                                        a.parent);
+  link_up<<<blocks, 256, 0, stream>>>(a.list, a.counter, a.capacity, a.width,
+                                      a.mask, a.prefix, a.wpr, a.parent);
+  flatten<<<blocks, 256, 0, stream>>>(a.counter, a.capacity, a.parent, a.roots);
+  launch_popcount_scan(a.roots, (static_cast<size_t>(a.capacity) + 63) / 32,
+                       a.counter, a.capacity, a.root_prefix, a.scan_tmp, stream);
+  number_components<<<blocks, 256, 0, stream>>>(a.list, a.counter, a.capacity,
+                                                a.parent, a.roots, a.root_prefix,
+                                                a.comp_of, a.comps, a.ncomp);
+  accumulate<<<blocks, 256, 0, stream>>>(a.list, a.counter, a.capacity, a.width,
+                                         a.comp_of, a.comps);
+  if (a.prev_mask) {
+    launch_fill(a.nlinks, 1, 0u, stream);
+    link_frames<<<blocks, 256, 0, stream>>>(
+        a.list, a.counter, a.capacity, a.width, a.comp_of, a.prev_mask,
+        a.prev_prefix, a.prev_comp_of, a.wpr, a.links, a.nlinks,
+        a.link_capacity);
+  }
+  check(hipGetLastError(), "launching the labelling");
+}
+
+} // namespace sfi

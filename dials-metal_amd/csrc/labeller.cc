@@ -1,0 +1,397 @@
+// Grouping of device-labelled frames into spots: the host side of the
+// reference's dials_spots::Labeller (src/spots/dials_spots.cc:36-360).
+//
+// The device hands over, per frame, the ordered signal pixels, each pixel's
+// four-connected component (numbered in order of first pixel) and each
+// component's exact integer moments, plus the pairs of components that share
+// a pixel position with the previous frame.  What is left is the part of
+// Labeller that works above the pixel level:
+//   * components of consecutive frames joined by a shared position are one
+//     three-dimensional component (six-connected, dials_spots.cc:128-137);
+//     a gap in frame numbers closes everything (:68-69);
+//   * after each frame, every component with nothing on the newest frame is
+//     emitted, in order of its first (frame, index) pixel (:153-219) -- in two
+//     dimensions every component of the frame at once;
+//   * emit applies the size filters, sums the moments into the bounding box,
+//     intensity, centroid at pixel centres, variance and peak, and applies the
+//     peak-centroid separation filter (:221-360).
+// Node-level union-find mirrors the reference's pixel-level one: nodes are
+// held in (frame, first index) order and merged toward the earlier, so labels
+// in order of first appearance are the reference's labels.
+//
+// Exactness.  Intensity, centroid and the filters come out identical to the
+// reference wherever its double sums are exact (every partial sum below 2^53,
+// true for counts below 2^32 on spots of a few thousand pixels): the moments
+// here are exact integers and each quantity is formed with the reference's
+// own final operations.  The variance's sum of value * delta^2 is formed from
+// the moments (N = S Q - W^2, exact in 128 bits) rather than summed pixel by
+// pixel, so it can differ from the reference's rounding in the last bits.
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <map>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#include "sf_labeller.hh"
+
+namespace sfl {
+
+namespace {
+
+constexpr uint32_t kNone = 0xFFFFFFFFu;
+constexpr double kPixelVariance = 1.0 / 12.0;
+using u128 = unsigned __int128;
+
+u128 pair128(const uint64_t p[2]) {
+  return (static_cast<u128>(p[1]) << 64) | p[0];
+}
+
+double to_double(u128 v) {
+  // Correctly rounded: split so that each part converts exactly, then one
+  // rounding in the addition when the value is below 2^106.
+  const uint64_t hi = static_cast<uint64_t>(v >> 64);
+  const uint64_t lo = static_cast<uint64_t>(v);
+  if (hi == 0)
+    return static_cast<double>(lo);
+  return std::ldexp(static_cast<double>(hi), 64) + static_cast<double>(lo);
+}
+
+} // namespace
+
+Labeller::Labeller(size_t height, size_t width, const Options &options)
+    : height_(height), width_(width), options_(options) {
+  if (height == 0 || width == 0)
+    throw std::runtime_error("dials_spots: a frame with no pixels in it");
+  if (options_.max_spot_size < options_.min_spot_size)
+    throw std::runtime_error("dials_spots: max_spot_size is below "
+                             "min_spot_size, so nothing can be accepted");
+  if (options_.min_spot_size == 0)
+    throw std::runtime_error("dials_spots: min_spot_size must be at least 1");
+}
+
+void Labeller::check_frame(int64_t frame, const sf_signal_pixel *signal,
+                           size_t n) const {
+  if (started_ && frame <= last_frame_)
+    throw std::runtime_error("dials_spots: frame " + std::to_string(frame) +
+                             " arrived after frame " +
+                             std::to_string(last_frame_) +
+                             "; frames must be added in ascending order");
+  const size_t pixels = height_ * width_;
+  for (size_t i = 0; i < n; i++) {
+    if (signal[i].index >= pixels)
+      throw std::runtime_error("dials_spots: signal pixel " +
+                               std::to_string(signal[i].index) +
+                               " is off the end of the frame");
+    if (i > 0 && signal[i].index <= signal[i - 1].index)
+      throw std::runtime_error(
+          "dials_spots: signal pixels are not ascending by index");
+  }
+}
+
+uint32_t Labeller::find(uint32_t i) {
+  while (parent_[i] != i) {
+    parent_[i] = parent_[parent_[i]];
+    i = parent_[i];
+  }
+  return i;
+}
+
+void Labeller::merge(uint32_t a, uint32_t b) {
+  const uint32_t ra = find(a), rb = find(b);
+  if (ra == rb)
+    return;
+  if (ra < rb)
+    parent_[rb] = ra;
+  else
+    parent_[ra] = rb;
+}
+
+void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
+                   const uint32_t *comp_of, const sf_component *comps,
+                   size_t ncomp, const uint32_t *links, size_t nlinks,
+                   bool links_to_previous) {
+  check_frame(frame, signal, n);
+  if (started_ && frame != last_frame_ + 1)
+    flush(frame);
+
+  // Pixels of this frame grouped by component, each group still ascending.
+  Arena &arena = arenas_[frame];
+  arena.index.resize(n);
+  arena.value.resize(n);
+  std::vector<uint32_t> start(ncomp + 1, 0);
+  for (size_t i = 0; i < n; i++) {
+    if (comp_of[i] >= ncomp)
+      throw std::runtime_error("device labelling: component out of range");
+    start[comp_of[i] + 1]++;
+  }
+  for (size_t c = 0; c < ncomp; c++)
+    start[c + 1] += start[c];
+  std::vector<uint32_t> at(start.begin(), start.end() - 1);
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t k = at[comp_of[i]]++;
+    arena.index[k] = signal[i].index;
+    arena.value[k] = signal[i].value;
+  }
+
+  const size_t base = nodes_.size();
+  const size_t previous = previous_;
+  for (size_t c = 0; c < ncomp; c++) {
+    Node node;
+    node.frame = frame;
+    node.c = comps[c];
+    node.begin = start[c];
+    node.end = start[c + 1];
+    if (node.end - node.begin != node.c.n)
+      throw std::runtime_error("device labelling: component sizes disagree");
+    nodes_.push_back(node);
+    parent_.push_back(static_cast<uint32_t>(nodes_.size() - 1));
+  }
+  counts_.signal_pixels += n;
+  live_pixels_ += n;
+
+  // Six-connected: components sharing a position with the previous frame.
+  if (!options_.two_d && started_ && last_frame_ + 1 == frame && links_to_previous) {
+    const size_t prev_count = base - previous;
+    for (size_t l = 0; l < nlinks; l++) {
+      const uint32_t here = links[2 * l], there = links[2 * l + 1];
+      if (here >= ncomp || there >= prev_count)
+        throw std::runtime_error("device labelling: a frame link out of range");
+      merge(static_cast<uint32_t>(base + here),
+            static_cast<uint32_t>(previous + there));
+    }
+  }
+  started_ = true;
+  last_frame_ = frame;
+  flush(options_.two_d ? frame + 1 : frame);
+}
+
+void Labeller::finish() {
+  if (!started_)
+    return;
+  flush(last_frame_ + 1);
+}
+
+void Labeller::flush(int64_t keep_frame) {
+  const size_t n = nodes_.size();
+  if (n == 0) {
+    previous_ = 0;
+    arenas_.clear();
+    return;
+  }
+  std::vector<uint32_t> label(n, kNone), root(n);
+  uint32_t components = 0;
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t r = find(static_cast<uint32_t>(i));
+    if (label[r] == kNone)
+      label[r] = components++;
+    root[i] = label[r];
+  }
+  std::vector<uint32_t> offset(components + 1, 0);
+  for (size_t i = 0; i < n; i++)
+    offset[root[i] + 1]++;
+  for (size_t c = 0; c < components; c++)
+    offset[c + 1] += offset[c];
+  std::vector<uint32_t> cursor(offset.begin(), offset.end() - 1);
+  std::vector<uint32_t> member(n);
+  for (size_t i = 0; i < n; i++)
+    member[cursor[root[i]]++] = static_cast<uint32_t>(i);
+  std::vector<char> open(components, 0);
+  for (size_t i = 0; i < n; i++)
+    if (nodes_[i].frame == keep_frame)
+      open[root[i]] = 1;
+  for (uint32_t c = 0; c < components; c++)
+    if (!open[c])
+      emit(&member[offset[c]], offset[c + 1] - offset[c]);
+
+  std::vector<uint32_t> first_kept(components, kNone);
+  size_t write = 0;
+  live_pixels_ = 0;
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t c = root[i];
+    if (!open[c])
+      continue;
+    if (first_kept[c] == kNone)
+      first_kept[c] = static_cast<uint32_t>(write);
+    nodes_[write] = nodes_[i];
+    parent_[write] = first_kept[c];
+    live_pixels_ += nodes_[write].c.n;
+    write++;
+  }
+  nodes_.resize(write);
+  parent_.resize(write);
+  previous_ = write;
+  while (previous_ > 0 && nodes_[previous_ - 1].frame == keep_frame)
+    previous_--;
+  // Pixels of frames that no open component touches are no longer needed.
+  for (auto it = arenas_.begin(); it != arenas_.end();) {
+    bool used = false;
+    for (size_t i = 0; i < write && !used; i++)
+      used = nodes_[i].frame == it->first;
+    it = used ? std::next(it) : arenas_.erase(it);
+  }
+}
+
+void Labeller::emit(const uint32_t *members, size_t count) {
+  counts_.groups++;
+  uint64_t n = 0;
+  for (size_t k = 0; k < count; k++)
+    n += nodes_[members[k]].c.n;
+  if (n < options_.min_spot_size) {
+    counts_.too_small++;
+    return;
+  }
+  if (n > options_.max_spot_size) {
+    counts_.too_large++;
+    return;
+  }
+
+  sf_spot spot;
+  std::memset(&spot, 0, sizeof(spot));
+  spot.n_signal = static_cast<uint32_t>(n);
+  u128 sum = 0, sum_sq = 0, wx = 0, wy = 0, wz = 0, qx = 0, qy = 0, qz = 0;
+  int64_t x0 = 0, x1 = 0, y0 = 0, y1 = 0, z0 = 0, z1 = 0;
+  uint32_t peak_value = 0;
+  int64_t peak_frame = 0;
+  uint32_t peak_index = 0;
+  bool have_peak = false;
+  for (size_t k = 0; k < count; k++) {
+    const Node &node = nodes_[members[k]];
+    const sf_component &c = node.c;
+    const int64_t z = node.frame;
+    if (k == 0) {
+      x0 = c.x0, x1 = c.x1, y0 = c.y0, y1 = c.y1, z0 = z, z1 = z + 1;
+    } else {
+      x0 = std::min<int64_t>(x0, c.x0);
+      x1 = std::max<int64_t>(x1, c.x1);
+      y0 = std::min<int64_t>(y0, c.y0);
+      y1 = std::max<int64_t>(y1, c.y1);
+      z0 = std::min(z0, z);
+      z1 = std::max(z1, z + 1);
+    }
+    const u128 s = c.sum;
+    const u128 az = static_cast<u128>(2 * z + 1);
+    sum += s;
+    sum_sq += pair128(c.sum_sq);
+    wx += pair128(c.sum_x);
+    wy += pair128(c.sum_y);
+    wz += az * s;
+    qx += pair128(c.sum_xx);
+    qy += pair128(c.sum_yy);
+    qz += az * az * s;
+    // The brightest pixel, ties to the lowest (frame, index): members come in
+    // (frame, first index) order, so a strictly-greater test across them
+    // matches the reference's (:279-284) once each component's own peak is
+    // its lowest-index brightest pixel.
+    const uint32_t value = static_cast<uint32_t>(c.peak >> 32);
+    const uint32_t index = 0xFFFFFFFFu - static_cast<uint32_t>(c.peak);
+    if (!have_peak || value > peak_value ||
+        (value == peak_value && (z < peak_frame || (z == peak_frame && index < peak_index)))) {
+      have_peak = true;
+      peak_value = value;
+      peak_frame = z;
+      peak_index = index;
+    }
+  }
+  spot.bbox[0] = static_cast<int32_t>(x0);
+  spot.bbox[1] = static_cast<int32_t>(x1);
+  spot.bbox[2] = static_cast<int32_t>(y0);
+  spot.bbox[3] = static_cast<int32_t>(y1);
+  spot.bbox[4] = static_cast<int32_t>(z0);
+  spot.bbox[5] = static_cast<int32_t>(z1);
+
+  const double sum_d = to_double(sum);
+  const double sum_sq_d = to_double(sum_sq);
+  spot.intensity = sum_d;
+  spot.intensity_variance = sum_d;
+  const double peak_pos[3] = {
+      static_cast<double>(peak_index % width_) + 0.5,
+      static_cast<double>(peak_index / width_) + 0.5,
+      static_cast<double>(peak_frame) + 0.5};
+  if (sum_d > 0.0) {
+    const u128 w[3] = {wx, wy, wz};
+    const u128 q[3] = {qx, qy, qz};
+    for (int a = 0; a < 3; a++)
+      spot.position[a] = (to_double(w[a]) * 0.5) / sum_d;
+    if (sum_d * sum_d > sum_sq_d) {
+      for (int a = 0; a < 3; a++) {
+        // sum of value * (pos - centroid)^2 = (S Q - W^2) / (4 S), with pos
+        // and W at doubled centres.
+        double delta_sq;
+        u128 sq, ww;
+        if (!__builtin_mul_overflow(sum, q[a], &sq) &&
+            !__builtin_mul_overflow(w[a], w[a], &ww)) {
+          delta_sq = to_double(sq - ww) / (4.0 * sum_d);
+        } else {
+          const long double S = static_cast<long double>(sum);
+          const long double W = static_cast<long double>(w[a]);
+          const long double Q = static_cast<long double>(q[a]);
+          delta_sq = static_cast<double>((S * Q - W * W) / (4.0L * S));
+        }
+        const double v = delta_sq * sum_d / (sum_d * sum_d - sum_sq_d);
+        spot.variance[a] = v / sum_d + kPixelVariance;
+      }
+    } else {
+      for (int a = 0; a < 3; a++)
+        spot.variance[a] = kPixelVariance;
+    }
+    if (spot.bbox[5] == spot.bbox[4] + 1)
+      spot.position[2] = static_cast<double>(spot.bbox[4]) + 0.5;
+  } else {
+    spot.position[0] = (spot.bbox[1] + spot.bbox[0]) / 2.0;
+    spot.position[1] = (spot.bbox[3] + spot.bbox[2]) / 2.0;
+    spot.position[2] = (spot.bbox[5] + spot.bbox[4]) / 2.0;
+  }
+  if (options_.max_separation > 0.0) {
+    const double dx = peak_pos[0] - spot.position[0];
+    const double dy = peak_pos[1] - spot.position[1];
+    const double dz = peak_pos[2] - spot.position[2];
+    if (std::sqrt(dx * dx + dy * dy + dz * dz) > options_.max_separation) {
+      counts_.separated++;
+      return;
+    }
+  }
+
+  // The spot's pixels in (frame, index) order: its components come in
+  // (frame, first) order, and those sharing a frame are merged by index.
+  spot.first = kept_.size();
+  size_t k = 0;
+  while (k < count) {
+    const int64_t z = nodes_[members[k]].frame;
+    size_t e = k;
+    while (e < count && nodes_[members[e]].frame == z)
+      e++;
+    const Arena &arena = arenas_.at(z);
+    std::vector<size_t> pos(e - k);
+    for (size_t j = k; j < e; j++)
+      pos[j - k] = nodes_[members[j]].begin;
+    for (;;) {
+      size_t best = kNone;
+      uint32_t best_index = 0;
+      for (size_t j = k; j < e; j++) {
+        const Node &node = nodes_[members[j]];
+        if (pos[j - k] < node.end) {
+          const uint32_t idx = arena.index[pos[j - k]];
+          if (best == kNone || idx < best_index) {
+            best = j;
+            best_index = idx;
+          }
+        }
+      }
+      if (best == kNone)
+        break;
+      sf_pixel p;
+      p.frame = static_cast<int32_t>(z);
+      p.index = best_index;
+      p.value = arena.value[pos[best - k]];
+      kept_.push_back(p);
+      pos[best - k]++;
+    }
+    k = e;
+  }
+  spots_.push_back(spot);
+  counts_.accepted++;
+}
+
+} // namespace sfl

@@ -45,9 +45,22 @@ __device__ __forceinline__ uint32_t block_inclusive(uint32_t v, uint32_t *total)
   return v + before;
 }
 
+// Words in use: all of them, or with a live bit count those that hold it
+// (none for a count above capacity, which the caller refuses anyway).
+__device__ __forceinline__ size_t live_words(size_t words, const uint32_t *live,
+                                             uint32_t capacity) {
+  if (live == nullptr)
+    return words;
+  const uint32_t n = *live;
+  const size_t w = n > capacity ? 0 : (static_cast<size_t>(n) + 31) / 32;
+  return w < words ? w : words;
+}
+
 // Per-block totals of popcounts.
 __global__ __launch_bounds__(kScanThreads) void popcount_totals(
-    const uint32_t *__restrict__ mask, size_t words, uint32_t *__restrict__ totals) {
+    const uint32_t *__restrict__ mask, size_t words, const uint32_t *live,
+    uint32_t capacity, uint32_t *__restrict__ totals) {
+  words = live_words(words, live, capacity);
   const size_t base = static_cast<size_t>(blockIdx.x) * kPerBlock +
                       static_cast<size_t>(threadIdx.x) * kPerThread;
   uint32_t c = 0;
@@ -78,8 +91,10 @@ __global__ __launch_bounds__(kScanThreads) void scan_totals(uint32_t *totals,
 
 // Exclusive prefix of popcounts per word.
 __global__ __launch_bounds__(kScanThreads) void popcount_prefix(
-    const uint32_t *__restrict__ mask, size_t words,
-    const uint32_t *__restrict__ totals, uint32_t *__restrict__ prefix) {
+    const uint32_t *__restrict__ mask, size_t words, const uint32_t *live,
+    uint32_t capacity, const uint32_t *__restrict__ totals,
+    uint32_t *__restrict__ prefix) {
+  words = live_words(words, live, capacity);
   const size_t base = static_cast<size_t>(blockIdx.x) * kPerBlock +
                       static_cast<size_t>(threadIdx.x) * kPerThread;
   uint32_t c[kPerThread];
@@ -118,6 +133,20 @@ __global__ __launch_bounds__(256) void scatter_by_rank(
   }
 }
 
+__global__ __launch_bounds__(256) void set_bits(
+    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
+    uint32_t capacity, uint32_t width, uint32_t *__restrict__ mask, uint32_t wpr) {
+  const uint32_t n = *counter;
+  if (n > capacity)
+    return;
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x) {
+    const uint32_t idx = list[i].index;
+    const uint32_t y = idx / width, x = idx - y * width;
+    atomicOr(&mask[static_cast<size_t>(y) * wpr + x / 32], 1u << (x & 31u));
+  }
+}
+
 __global__ __launch_bounds__(256) void fill_words(uint32_t *p, size_t n,
                                                   uint32_t value) {
   for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
@@ -129,28 +158,48 @@ __global__ __launch_bounds__(256) void fill_words(uint32_t *p, size_t n,
 
 namespace sfi {
 
-size_t order_scan_bytes(size_t words) {
+size_t scan_tmp_bytes(size_t words) {
   return ((words + kPerBlock - 1) / kPerBlock + 1) * sizeof(uint32_t);
+}
+
+size_t order_scan_bytes(size_t words) { return scan_tmp_bytes(words); }
+
+void launch_popcount_scan(const uint32_t *bits, size_t words,
+                          const uint32_t *live, uint32_t capacity,
+                          uint32_t *prefix, void *tmp, hipStream_t stream) {
+  const uint32_t blocks =
+      static_cast<uint32_t>((words + kPerBlock - 1) / kPerBlock);
+  if (blocks == 0)
+    return;
+  uint32_t *totals = static_cast<uint32_t *>(tmp);
+  popcount_totals<<<blocks, kScanThreads, 0, stream>>>(bits, words, live,
+                                                       capacity, totals);
+  scan_totals<<<1, kScanThreads, 0, stream>>>(totals, blocks);
+  popcount_prefix<<<blocks, kScanThreads, 0, stream>>>(bits, words, live,
+                                                       capacity, totals, prefix);
+  check(hipGetLastError(), "launching a popcount scan");
 }
 
 void launch_order(const FrameArgs &a, const sf_signal_pixel *unordered,
                   sf_signal_pixel *ordered, uint32_t *prefix, void *scan_tmp,
                   size_t scan_tmp_bytes, hipStream_t stream) {
   const size_t words = static_cast<size_t>(a.height) * a.wpr;
-  const uint32_t blocks = static_cast<uint32_t>((words + kPerBlock - 1) / kPerBlock);
   if (scan_tmp_bytes < order_scan_bytes(words))
     throw device_error("the rank scan's scratch is too small");
-  uint32_t *totals = static_cast<uint32_t *>(scan_tmp);
-  popcount_totals<<<blocks, kScanThreads, 0, stream>>>(a.mask, words, totals);
-  scan_totals<<<1, kScanThreads, 0, stream>>>(totals, blocks);
-  popcount_prefix<<<blocks, kScanThreads, 0, stream>>>(a.mask, words, totals,
-                                                       prefix);
+  launch_popcount_scan(a.mask, words, nullptr, 0, prefix, scan_tmp, stream);
   // Enough blocks to cover a dense frame in a few strides; the count itself
   // is only known on the device.
   scatter_by_rank<<<1024, 256, 0, stream>>>(unordered, ordered, a.counter,
                                             a.capacity, a.mask, prefix,
                                             a.width, a.wpr);
   check(hipGetLastError(), "launching the rank scatter");
+}
+
+void launch_set_bits(const sf_signal_pixel *list, const uint32_t *counter,
+                     uint32_t capacity, uint32_t width, uint32_t *mask,
+                     uint32_t wpr, hipStream_t stream) {
+  set_bits<<<512, 256, 0, stream>>>(list, counter, capacity, width, mask, wpr);
+  check(hipGetLastError(), "launching set_bits");
 }
 
 void launch_fill(uint32_t *p, size_t n, uint32_t value, hipStream_t stream) {

@@ -65,4 +65,43 @@ void launch_order(const FrameArgs &a, const sf_signal_pixel *unordered,
 // correct when the sequence is captured into a graph and replayed.
 void launch_fill(uint32_t *p, size_t n, uint32_t value, hipStream_t stream);
 
+// Exclusive prefix of per-word popcounts over `words` words.  With `live`
+// (a device count of bits in use, refused as zero above `capacity`), words
+// past the live bits are read as zero.  `tmp` needs scan_tmp_bytes(words).
+size_t scan_tmp_bytes(size_t words);
+void launch_popcount_scan(const uint32_t *bits, size_t words,
+                          const uint32_t *live, uint32_t capacity,
+                          uint32_t *prefix, void *tmp, hipStream_t stream);
+
+// One frame's labelling: the ordered list and its bitmask in, components out.
+struct LabelArgs {
+  const sf_signal_pixel *list; // ordered, `*counter` entries
+  const uint32_t *counter;
+  uint32_t capacity;
+  uint32_t width;
+  const uint32_t *mask, *prefix; // the frame's bitmask and word prefix
+  uint32_t wpr;
+  uint32_t *parent;      // capacity
+  uint32_t *roots;       // capacity / 32 + 2 words
+  uint32_t *root_prefix; // same
+  void *scan_tmp;        // scan_tmp_bytes(capacity / 32 + 2)
+  uint32_t *comp_of;     // capacity
+  sf_component *comps;   // capacity
+  uint32_t *ncomp;       // device word
+  // The previous frame of the same shape, for three-dimensional links, or
+  // null.
+  const uint32_t *prev_mask, *prev_prefix, *prev_comp_of;
+  uint32_t *links;       // 2 * link_capacity words
+  uint32_t *nlinks;      // device word
+  uint32_t link_capacity;
+};
+
+void launch_label(const LabelArgs &a, hipStream_t stream);
+
+// Sets the bitmask bit of every pixel of an ordered list of `*counter`
+// entries (at most `capacity`); the mask starts clear.
+void launch_set_bits(const sf_signal_pixel *list, const uint32_t *counter,
+                     uint32_t capacity, uint32_t width, uint32_t *mask,
+                     uint32_t wpr, hipStream_t stream);
+
 } // namespace sfi

@@ -1,0 +1,81 @@
+// The host side of the grouping: components labelled on the device, joined
+// across frames and reduced to spots as dials_spots::Labeller does
+// (src/spots/dials_spots.hh:108-158).  Not part of the public ABI; the C
+// functions sf_labeller_* in include/spotfinder_hip.h wrap it.
+#pragma once
+
+#include <cstddef>
+#include <cstdint>
+#include <map>
+#include <vector>
+
+#include "spotfinder_hip.h"
+
+namespace sfl {
+
+// dials_spots::Options (src/spots/dials_spots.hh:81-95).
+struct Options {
+  size_t min_spot_size = 3;
+  size_t max_spot_size = 1000;
+  double max_separation = 2.0;
+  bool two_d = false;
+};
+
+class Labeller {
+public:
+  Labeller(size_t height, size_t width, const Options &options);
+
+  // Validates a frame's list as Labeller::add does before grouping it
+  // (dials_spots.cc:57-98): throws std::runtime_error.
+  void check_frame(int64_t frame, const sf_signal_pixel *signal,
+                   size_t n) const;
+
+  // One device-labelled frame: the ordered signal list, each pixel's
+  // component, the components, and (frame-1 components) links.  The links
+  // are honoured only when `links_to_previous` says they were taken against
+  // the frame added last.
+  void add(int64_t frame, const sf_signal_pixel *signal, size_t n,
+           const uint32_t *comp_of, const sf_component *comps, size_t ncomp,
+           const uint32_t *links, size_t nlinks, bool links_to_previous);
+  void finish();
+
+  bool started() const { return started_; }
+  int64_t last_frame() const { return last_frame_; }
+  const std::vector<sf_spot> &spots() const { return spots_; }
+  const std::vector<sf_pixel> &pixels() const { return kept_; }
+  const sf_counts &counts() const { return counts_; }
+  size_t live() const { return live_pixels_; }
+  size_t height() const { return height_; }
+  size_t width() const { return width_; }
+  const Options &options() const { return options_; }
+
+private:
+  struct Node {
+    int64_t frame;
+    sf_component c;
+    uint32_t begin, end; // pixel range in the frame's arena
+  };
+  struct Arena {
+    std::vector<uint32_t> index, value;
+  };
+
+  uint32_t find(uint32_t i);
+  void merge(uint32_t a, uint32_t b);
+  void flush(int64_t keep_frame);
+  void emit(const uint32_t *members, size_t count);
+
+  size_t height_, width_;
+  Options options_;
+  std::vector<Node> nodes_;      // open components, (frame, first) order
+  std::vector<uint32_t> parent_;
+  size_t previous_ = 0;          // where the last frame's nodes start
+  std::map<int64_t, Arena> arenas_;
+  int64_t last_frame_ = 0;
+  bool started_ = false;
+  size_t live_pixels_ = 0;
+  std::vector<sf_spot> spots_;
+  std::vector<sf_pixel> kept_;
+  sf_counts counts_{};
+};
+
+} // namespace sfl

@@ -30,7 +30,24 @@ SPOT = np.dtype(
 COUNTS_FIELDS = ("signal_pixels", "groups", "too_small", "too_large",
                  "separated", "accepted")
 
+COMPONENT = np.dtype(
+    [("first", "<u4"), ("n", "<u4"), ("x0", "<u4"), ("x1", "<u4"),
+     ("y0", "<u4"), ("y1", "<u4"), ("peak", "<u8"), ("sum", "<u8"),
+     ("sum_sq", "<u8", (2,)), ("sum_x", "<u8", (2,)), ("sum_y", "<u8", (2,)),
+     ("sum_xx", "<u8", (2,)), ("sum_yy", "<u8", (2,))])
+assert COMPONENT.itemsize == 120
+
 KERNELS = {"fused": 0, "direct": 1}
+
+
+class Grouping(ctypes.Structure):
+    _fields_ = [("min_spot_size", ctypes.c_size_t),
+                ("max_spot_size", ctypes.c_size_t),
+                ("max_separation", ctypes.c_double), ("two_d", ctypes.c_int)]
+
+
+class Counts(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in COUNTS_FIELDS]
 
 
 class SpotfinderError(RuntimeError):
@@ -84,6 +101,11 @@ def lib():
         "sf_host_alloc": ([sz], vp),
         "sf_host_free": ([vp], None),
         "sf_last_error": ([], ctypes.c_char_p),
+        "sf_device_alloc": ([sz], vp),
+        "sf_device_free": ([vp], None),
+        "sf_copy_to_device": ([vp, vp, sz], ctypes.c_int),
+        "sf_copy_to_host": ([vp, vp, sz], ctypes.c_int),
+        "sf_device_synchronize": ([], ctypes.c_int),
         "sf_set_kernel": ([ctypes.c_int], None),
         "sf_get_kernel": ([], ctypes.c_int),
         "sf_profile_stages": ([ctypes.c_int], None),
@@ -103,6 +125,27 @@ def lib():
                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "sf_ws_fallbacks": ([vp], ctypes.c_uint64),
         "sf_ws_device_mask": ([vp, ctypes.POINTER(u32)], vp),
+        "sf_ws_label": ([vp, ctypes.c_int], ctypes.c_int),
+        "sf_ws_sync_label": ([vp, ctypes.POINTER(sz), ctypes.POINTER(sz),
+                              ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_int)],
+                             ctypes.c_int),
+        "sf_ws_fetch_labels": ([vp, vp, sz, vp, sz, vp, sz], ctypes.c_int),
+        "sf_ws_load_signal": ([vp, vp, sz, u32, u32], ctypes.c_int),
+        "sf_grouping_default": ([ctypes.POINTER(Grouping)], None),
+        "sf_labeller_create": ([sz, sz, ctypes.POINTER(Grouping),
+                                ctypes.POINTER(vp)], ctypes.c_int),
+        "sf_labeller_destroy": ([vp], None),
+        "sf_labeller_add": ([vp, ctypes.c_int64, vp, sz], ctypes.c_int),
+        "sf_labeller_add_ws": ([vp, ctypes.c_int64, vp], ctypes.c_int),
+        "sf_labeller_add_labelled": ([vp, ctypes.c_int64, vp, sz, vp, vp, sz,
+                                      vp, sz, ctypes.c_int], ctypes.c_int),
+        "sf_labeller_finish": ([vp], ctypes.c_int),
+        "sf_labeller_spot_count": ([vp], sz),
+        "sf_labeller_spots": ([vp], vp),
+        "sf_labeller_pixel_count": ([vp], sz),
+        "sf_labeller_pixels": ([vp], vp),
+        "sf_labeller_counts": ([vp], Counts),
+        "sf_labeller_live": ([vp], sz),
         "sf_synth_default": ([ctypes.POINTER(SynthDesc), u32, u32], None),
         "sf_synth_eiger": ([ctypes.POINTER(SynthDesc), ctypes.c_int, ctypes.c_int], None),
         "sf_synth_frame": ([ctypes.POINTER(SynthDesc), ctypes.c_int64, vp], ctypes.c_int),
@@ -142,6 +185,51 @@ def unavailable_reason():
 
 def memory_free():
     return int(lib().sf_memory_free())
+
+
+def synchronize():
+    _check(lib().sf_device_synchronize())
+
+
+class DeviceArray:
+    """A device allocation made through the library's own runtime (one GPU
+    runtime per process: a second, such as a framework's bundled one, cannot
+    open a device this one already holds)."""
+
+    def __init__(self, nbytes):
+        self.nbytes = int(nbytes)
+        self.ptr = lib().sf_device_alloc(self.nbytes)
+        if not self.ptr:
+            raise SpotfinderError(-4, last_error())
+
+    @classmethod
+    def from_numpy(cls, a):
+        a = np.ascontiguousarray(a)
+        d = cls(a.nbytes)
+        _check(lib().sf_copy_to_device(d.ptr, _ptr(a), a.nbytes))
+        return d
+
+    def copy_from(self, a):
+        a = np.ascontiguousarray(a)
+        if a.nbytes > self.nbytes:
+            raise ValueError("array larger than the allocation")
+        _check(lib().sf_copy_to_device(self.ptr, _ptr(a), a.nbytes))
+
+    def to_numpy(self, dtype, shape):
+        out = np.empty(shape, dtype)
+        _check(lib().sf_copy_to_host(_ptr(out), self.ptr, out.nbytes))
+        return out
+
+    def free(self):
+        if self.ptr:
+            lib().sf_device_free(self.ptr)
+            self.ptr = None
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
 
 
 def set_kernel(name):
@@ -275,3 +363,105 @@ def synth_frame(desc, frame=0, out=None):
     if rc != 0:
         raise ValueError("bad synthetic frame description")
     return out
+
+
+class LabellerError(RuntimeError):
+    """dials_spots::Labeller's std::runtime_error, through the C ABI."""
+
+
+class Labeller:
+    """dials_spots::Labeller (src/spots/dials_spots.hh:108-158), with the
+    labelling and per-spot reductions on the device."""
+
+    def __init__(self, height, width, min_spot_size=3, max_spot_size=1000,
+                 max_separation=2.0, two_d=False):
+        g = Grouping()
+        lib().sf_grouping_default(ctypes.byref(g))
+        g.min_spot_size, g.max_spot_size = min_spot_size, max_spot_size
+        g.max_separation, g.two_d = max_separation, int(bool(two_d))
+        self.h = ctypes.c_void_p()
+        if lib().sf_labeller_create(height, width, ctypes.byref(g),
+                                    ctypes.byref(self.h)) != 0:
+            raise LabellerError(last_error())
+
+    def __del__(self):
+        try:
+            if self.h:
+                lib().sf_labeller_destroy(self.h)
+                self.h = ctypes.c_void_p()
+        except Exception:
+            pass
+
+    def _status(self, rc):
+        if rc == -1:
+            raise LabellerError(last_error())
+        _check(rc)
+
+    def add(self, frame, signal):
+        signal = np.ascontiguousarray(signal, SIGNAL_PIXEL)
+        self._status(lib().sf_labeller_add(self.h, int(frame), _ptr(signal),
+                                           len(signal)))
+
+    def add_workspace(self, frame, ws):
+        self._status(lib().sf_labeller_add_ws(self.h, int(frame), ws.h))
+
+    def add_labelled(self, frame, signal, comp_of, comps, links,
+                     links_to_previous=True):
+        signal = np.ascontiguousarray(signal, SIGNAL_PIXEL)
+        comp_of = np.ascontiguousarray(comp_of, np.uint32)
+        comps = np.ascontiguousarray(comps, COMPONENT)
+        links = np.ascontiguousarray(links, np.uint32).reshape(-1)
+        self._status(lib().sf_labeller_add_labelled(
+            self.h, int(frame), _ptr(signal), len(signal), _ptr(comp_of),
+            _ptr(comps), len(comps), _ptr(links), len(links) // 2,
+            int(bool(links_to_previous))))
+
+    def finish(self):
+        _check(lib().sf_labeller_finish(self.h))
+
+    def spots(self):
+        n = lib().sf_labeller_spot_count(self.h)
+        out = np.zeros(n, SPOT)
+        if n:
+            ctypes.memmove(_ptr(out), lib().sf_labeller_spots(self.h),
+                           n * SPOT.itemsize)
+        return out
+
+    def pixels(self):
+        n = lib().sf_labeller_pixel_count(self.h)
+        out = np.zeros(n, PIXEL)
+        if n:
+            ctypes.memmove(_ptr(out), lib().sf_labeller_pixels(self.h),
+                           n * PIXEL.itemsize)
+        return out
+
+    def counts(self):
+        c = lib().sf_labeller_counts(self.h)
+        return {f: int(getattr(c, f)) for f in COUNTS_FIELDS}
+
+    def live(self):
+        return int(lib().sf_labeller_live(self.h))
+
+
+def workspace_label(ws, link_previous=True):
+    """Labels the workspace's last frame; returns (comp_of, comps, links,
+    linked) copied to the host."""
+    _check(lib().sf_ws_label(ws.h, int(bool(link_previous))))
+    n, nc, nl = ctypes.c_size_t(0), ctypes.c_size_t(0), ctypes.c_size_t(0)
+    linked = ctypes.c_int(0)
+    _check(lib().sf_ws_sync_label(ws.h, ctypes.byref(n), ctypes.byref(nc),
+                                  ctypes.byref(nl), ctypes.byref(linked)))
+    comp_of = np.zeros(n.value, np.uint32)
+    comps = np.zeros(nc.value, COMPONENT)
+    links = np.zeros(2 * nl.value, np.uint32)
+    _check(lib().sf_ws_fetch_labels(ws.h, _ptr(comp_of), n.value, _ptr(comps),
+                                    nc.value, _ptr(links), nl.value))
+    return comp_of, comps, links.reshape(-1, 2), bool(linked.value)
+
+
+def workspace_load_signal(ws, signal, height, width):
+    signal = np.ascontiguousarray(signal, SIGNAL_PIXEL)
+    rc = lib().sf_ws_load_signal(ws.h, _ptr(signal), len(signal), height, width)
+    if rc == -1:
+        raise ValueError(last_error())
+    _check(rc)

@@ -70,8 +70,9 @@ typedef struct sf_component {
   uint32_t n;         /* pixels */
   uint32_t x0, x1;    /* bbox columns, x1 exclusive */
   uint32_t y0, y1;    /* bbox rows, y1 exclusive */
-  uint32_t peak_value;
-  uint32_t peak_index; /* lowest index among the brightest pixels */
+  /* The brightest pixel, ties to the lowest index:
+   * (value << 32) | (0xFFFFFFFF - index). */
+  uint64_t peak;
   uint64_t sum;        /* sum of values */
   uint64_t sum_sq[2];  /* sum of value^2, lo/hi */
   uint64_t sum_x[2];   /* sum of value*(2x+1), lo/hi */
@@ -92,6 +93,14 @@ size_t sf_memory_free(void);               /* device bytes free, 0 on error */
 void *sf_host_alloc(size_t bytes);         /* pinned; NULL on failure */
 void sf_host_free(void *pointer);
 const char *sf_last_error(void);           /* this thread's last failure */
+
+/* Device memory for callers that hold frames in HBM, so that a binding
+ * needs no second GPU runtime of its own.  Copies are synchronous; 0 or -3. */
+void *sf_device_alloc(size_t bytes);       /* NULL on failure */
+void sf_device_free(void *pointer);
+int sf_copy_to_device(void *dst, const void *src, size_t bytes);
+int sf_copy_to_host(void *dst, const void *src, size_t bytes);
+int sf_device_synchronize(void);
 
 /* ------------------------------------------------------------------------
  * Kernel choice: replaces gpu::stage0_window/stage2_window/fused
@@ -160,9 +169,76 @@ int sf_ws_kernel_ms(sf_workspace *ws, double *threshold_ms, double *order_ms);
 /* Frames this workspace's fused kernel handed back to the direct kernels
  * because a valid pixel was beyond its exact fast range. */
 uint64_t sf_ws_fallbacks(sf_workspace *ws);
+/* Labelling of the last thresholded frame, enqueued on the workspace's
+ * stream: four-connected components numbered in order of their first pixel,
+ * each pixel's component, each component's exact moments (sf_component),
+ * and, with link_previous, the (component here, component on the previous
+ * frame) pairs of positions set in both -- taken only when the workspace's
+ * previous frame was thresholded and labelled just before and has the same
+ * shape.  New surface: the reference labels on the host only
+ * (dials_spots.cc:57-145). */
+int sf_ws_label(sf_workspace *ws, int link_previous);
+/* Waits; pixel, component and link counts.  *linked says whether links were
+ * taken. */
+int sf_ws_sync_label(sf_workspace *ws, size_t *n, size_t *ncomp,
+                     size_t *nlinks, int *linked);
+int sf_ws_fetch_labels(sf_workspace *ws, uint32_t *comp_of, size_t n,
+                       sf_component *comps, size_t ncomp, uint32_t *links,
+                       size_t nlinks);
+/* Loads an ordered host list (as a threshold would leave it) so that it can
+ * be labelled: -1 if it is not ascending or runs off an h x w frame. */
+int sf_ws_load_signal(sf_workspace *ws, const sf_signal_pixel *signal,
+                      size_t n, uint32_t height, uint32_t width);
 /* Device pointers of the last frame's results. */
 const sf_signal_pixel *sf_ws_device_signal(sf_workspace *ws);
 const uint32_t *sf_ws_device_mask(sf_workspace *ws, uint32_t *words_per_row);
+
+/* ------------------------------------------------------------------------
+ * Grouping into spots: dials_spots::Labeller (src/spots/dials_spots.hh:
+ * 108-158) with the pixel-level labelling and reductions on the device.
+ *
+ * sf_labeller_add is Labeller::add: one frame's ascending signal list; frames
+ * strictly ascending; a gap in numbering closes every open spot; -1 with the
+ * reference's message for a frame out of order, a pixel off the frame or
+ * pixels out of order.  sf_labeller_add_ws adds the frame a workspace has
+ * just thresholded and labelled (sf_ws_threshold, sf_ws_label), without
+ * copying its list back first.  Spots, pixels and counts are the reference's
+ * records, in its order.
+ * ------------------------------------------------------------------------ */
+
+/* dials_spots::Options (src/spots/dials_spots.hh:81-95). */
+typedef struct sf_grouping {
+  size_t min_spot_size;   /* 3 */
+  size_t max_spot_size;   /* 1000 */
+  double max_separation;  /* 2.0; 0 or less turns the filter off */
+  int two_d;              /* 0: six-connected across frames */
+} sf_grouping;
+void sf_grouping_default(sf_grouping *g);
+
+typedef struct sf_labeller sf_labeller;
+int sf_labeller_create(size_t height, size_t width, const sf_grouping *g,
+                       sf_labeller **out);
+void sf_labeller_destroy(sf_labeller *l);
+int sf_labeller_add(sf_labeller *l, int64_t frame,
+                    const sf_signal_pixel *signal, size_t n);
+int sf_labeller_add_ws(sf_labeller *l, int64_t frame, sf_workspace *ws);
+/* A frame labelled elsewhere (a batched launch, another device): its
+ * ordered list, each pixel's component (numbered in order of first pixel),
+ * the components, and (here, previous-frame) component pairs, honoured when
+ * links_to_previous is set and the frame follows the last one added. */
+int sf_labeller_add_labelled(sf_labeller *l, int64_t frame,
+                             const sf_signal_pixel *signal, size_t n,
+                             const uint32_t *comp_of,
+                             const sf_component *comps, size_t ncomp,
+                             const uint32_t *links, size_t nlinks,
+                             int links_to_previous);
+int sf_labeller_finish(sf_labeller *l);
+size_t sf_labeller_spot_count(const sf_labeller *l);
+const sf_spot *sf_labeller_spots(const sf_labeller *l);
+size_t sf_labeller_pixel_count(const sf_labeller *l);
+const sf_pixel *sf_labeller_pixels(const sf_labeller *l);
+sf_counts sf_labeller_counts(const sf_labeller *l);
+size_t sf_labeller_live(const sf_labeller *l);
 
 #ifdef __cplusplus
 }

@@ -1,0 +1,90 @@
+"""The device labelling and the grouping built on it, on the device.
+
+  * the device's components, labels, moments and frame links equal the
+    plain-Python model's exactly (tests/labelling_model.py);
+  * sf.Labeller (device labelling + host grouping) passes the reference
+    suite's hand-derived cases and reproduces the reference Labeller's
+    golden sweep: spots exact but for the variance (1e-9), pixels and counts
+    exact, in 3D and 2D;
+  * threshold -> label -> group on full-size EIGER2 frames from HBM equals
+    the oracle's threshold grouped by the oracle's Labeller.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+from labeller_cases import compare_spots, run_cases
+from labelling_model import label_frame, links_between
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def test_device_labels_equal_the_model(device, sweep):
+    ws = device.Workspace()
+    prev = None
+    for z in range(10):
+        sig = sweep[f"signal_{z}"]
+        say("label frame", z, len(sig))
+        device.workspace_load_signal(ws, sig, 384, 320)
+        comp_of, comps, links, linked = device.workspace_label(ws, True)
+        m_comp_of, m_comps, m_rank = label_frame(sig, 320)
+        assert (comp_of == m_comp_of).all()
+        assert comps.tobytes() == m_comps.tobytes()
+        assert linked == (z > 0)
+        if prev is not None:
+            want = links_between(sig, m_comp_of, prev[0], prev[1])
+            got = np.unique(links, axis=0) if len(links) else links
+            assert got.tolist() == want.tolist()
+        prev = (m_rank, m_comp_of)
+    ws.close()
+
+
+def test_device_labeller_cases(device):
+    run_cases(lambda h, w, **kw: device.Labeller(h, w, **kw),
+              device.LabellerError)
+
+
+@pytest.mark.parametrize("mode", ["3d", "2d"])
+def test_device_labeller_golden_sweep(device, sweep, mode):
+    lab = device.Labeller(384, 320, two_d=(mode == "2d"))
+    for z in range(10):
+        lab.add(z, sweep[f"signal_{z}"])
+    lab.finish()
+    compare_spots(lab.spots(), sweep[f"spots_{mode}"])
+    assert lab.pixels().tobytes() == sweep[f"pixels_{mode}"].tobytes()
+    c = lab.counts()
+    assert [c[k] for k in O.COUNTS_FIELDS] == list(sweep[f"counts_{mode}"])
+
+
+@pytest.mark.parametrize("mode", ["3d", "2d"])
+def test_threshold_label_group_from_hbm(device, mode):
+    desc = device.synth_desc("4m", spots_per_frame=600, persist=1, seed=77)
+    h, w = desc.height, desc.width
+    ws = device.Workspace()
+    lab = device.Labeller(h, w, two_d=(mode == "2d"))
+    ref = O.oracle().labeller(h, w, two_d=(mode == "2d"))
+    for z in range(5):
+        frame = device.synth_frame(desc, z)
+        d = device.DeviceArray.from_numpy(frame)
+        say(mode, "frame", z)
+        ws.threshold(d.ptr, 32, h, w, "fused")
+        rc, n = ws.sync()
+        assert rc == 0
+        _ = device.workspace_label(ws, True)
+        lab.add_workspace(z, ws)
+        _, want = O.oracle().dext(frame)
+        ref.add(z, want)
+    lab.finish()
+    ref.finish()
+    compare_spots(lab.spots(), ref.spots())
+    assert lab.pixels().tobytes() == ref.pixels().tobytes()
+    assert lab.counts() == ref.counts()
+    assert len(lab.spots()) > 1000
+    ws.close()

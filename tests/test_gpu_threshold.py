@@ -105,15 +105,14 @@ def test_bad_arguments(device):
 
 
 def test_device_workspace_matches_find(device, synth_cases):
-    import torch
     case = synth_cases["eiger4m_f0"]
     frame = device.synth_frame(device.synth_desc("4m", **case["overrides"]),
                                case["frame"])
-    d = torch.from_numpy(frame.view(np.int32)).to("cuda")
+    d = device.DeviceArray.from_numpy(frame)
     ws = device.Workspace()
     for kernel in KERNELS:
         say("workspace", kernel)
-        ws.threshold(d.data_ptr(), 32, frame.shape[0], frame.shape[1], kernel)
+        ws.threshold(d.ptr, 32, frame.shape[0], frame.shape[1], kernel)
         rc, n = ws.sync()
         assert rc == 0 and n == case["count"]
         got = ws.fetch(n)
@@ -130,3 +129,16 @@ def test_stage_times(device):
         assert rc == 0 and t["upload"] > 0 and t["fused"] > 0
     finally:
         device.profile_stages(False)
+
+
+def test_cpp_dropin_interface(device):
+    """gpu::find<T> through include/dext_gpu.hh, built by the package
+    Makefile as tests/cpp/test_dext_gpu, against the oracle."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "tests", "cpp", "test_dext_gpu")
+    assert os.path.exists(exe), "build first: make -C dials-metal_amd tests"
+    r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
+    print(r.stdout[-2000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr

@@ -1,0 +1,99 @@
+"""The grouping, without a device.
+
+The oracle's Labeller and the reference's own (oracle/_ref) against the
+reference test suite's hand-derived cases and the golden sweep; then the
+library's host-side grouping (sf_labeller_add_labelled), fed the device's
+contract by a plain-Python model of the per-frame labelling, against the same
+cases and the reference's golden output."""
+import numpy as np
+import pytest
+
+from labeller_cases import compare_spots, run_cases
+from labelling_model import ModelFeeder
+from oracle import oracle as O
+
+
+def oracle_make(h, w, **kw):
+    return O.oracle().labeller(h, w, **kw)
+
+
+def test_oracle_labeller_cases():
+    run_cases(oracle_make, O.LabellerError)
+
+
+def test_reference_labeller_cases():
+    ref = O.reference()
+    if ref is None:
+        pytest.skip("oracle/_ref is not built (no reference sources here)")
+    run_cases(lambda h, w, **kw: ref.labeller(h, w, **kw), O.LabellerError)
+
+
+@pytest.mark.parametrize("mode", ["3d", "2d"])
+def test_oracle_labeller_matches_golden_sweep(sweep, mode):
+    lab = O.oracle().labeller(384, 320, two_d=(mode == "2d"))
+    for z in range(10):
+        lab.add(z, sweep[f"signal_{z}"])
+    lab.finish()
+    got = lab.spots()
+    want = sweep[f"spots_{mode}"]
+    # The oracle sums in the reference's own order: bit-exact, variance too
+    # (field by field: the record's padding is not initialised).
+    compare_spots(got, want, tol=0.0)
+    assert lab.pixels().tobytes() == sweep[f"pixels_{mode}"].tobytes()
+    c = lab.counts()
+    assert [c[k] for k in O.COUNTS_FIELDS] == list(sweep[f"counts_{mode}"])
+
+
+class ModelLabeller:
+    """sf.Labeller driven through the device contract by the Python model."""
+
+    def __init__(self, h, w, **kw):
+        import spotfinder_hip as sf
+        self.lab = sf.Labeller(h, w, **kw)
+        self.feed = ModelFeeder(self.lab, w)
+        self.h, self.w = h, w
+
+    def add(self, frame, signal):
+        import spotfinder_hip as sf
+        signal = np.ascontiguousarray(signal, sf.SIGNAL_PIXEL)
+        idx = signal["index"].astype(np.int64)
+        if len(idx) and (idx.max() >= self.h * self.w or (np.diff(idx) <= 0).any()):
+            # what the device path refuses before labelling
+            raise sf.LabellerError("bad pixel list")
+        self.feed.add(frame, signal)
+
+    def __getattr__(self, name):
+        return getattr(self.lab, name)
+
+
+def test_host_grouping_cases(sf):
+    run_cases(lambda h, w, **kw: ModelLabeller(h, w, **kw), sf.LabellerError)
+
+
+@pytest.mark.parametrize("mode", ["3d", "2d"])
+def test_host_grouping_matches_golden_sweep(sf, sweep, mode):
+    lab = ModelLabeller(384, 320, two_d=(mode == "2d"))
+    for z in range(10):
+        lab.add(z, sweep[f"signal_{z}"])
+    lab.finish()
+    compare_spots(lab.spots(), sweep[f"spots_{mode}"])
+    assert lab.pixels().tobytes() == sweep[f"pixels_{mode}"].tobytes()
+    c = lab.counts()
+    assert [c[k] for k in O.COUNTS_FIELDS] == list(sweep[f"counts_{mode}"])
+
+
+def test_host_grouping_refuses_out_of_order_frames(sf):
+    lab = sf.Labeller(8, 10)
+    feed = ModelFeeder(lab, 10)
+    feed.add(3, np.zeros(0, sf.SIGNAL_PIXEL))
+    with pytest.raises(sf.LabellerError):
+        feed.add(3, np.zeros(0, sf.SIGNAL_PIXEL))
+
+
+def test_bad_grouping_options(sf):
+    with pytest.raises(sf.LabellerError):
+        sf.Labeller(0, 10)
+    with pytest.raises(sf.LabellerError):
+        sf.Labeller(8, 10, min_spot_size=5, max_spot_size=4)
+    with pytest.raises(sf.LabellerError):
+        sf.Labeller(8, 10, min_spot_size=0)
