@@ -1,11 +1,11 @@
-// Strong-pixel connected components of one frame, and their exact moments,
-// on the device.
+// Strong-pixel connected components of each frame of a batch, and their
+// exact moments, on the device.
 //
 // The reference groups on the host (dials_spots::Labeller::add,
 // src/spots/dials_spots.cc:57-145): pixels are unioned with their left
 // neighbour (same row) and the one above, toward the lower position, and a
 // component's rows are emitted in order of its first pixel.  Here the same
-// forest is built in the rank space of the frame's ordered signal list:
+// forest is built in the rank space of each frame's ordered signal list:
 //   * the left neighbour of rank i, when set, is rank i - 1; the one above is
 //     found from the strong-pixel bitmask and its word prefix;
 //   * a lock-free union-find hooks the larger root under the smaller, so the
@@ -16,10 +16,15 @@
 //     atomics (128-bit where a product can exceed 64 bits).
 // Links to the previous frame (same index set in both frames) are listed for
 // the host's three-dimensional stitch (dials_spots.cc:128-137).
+//
+// Every kernel works on frame blockIdx.y of a descriptor array, so a batch
+// is labelled in the same few launches as one frame.
 #include "sf_device.hh"
 #include "sf_internal.hh"
 
 namespace {
+
+using sfi::FrameDev;
 
 __device__ __forceinline__ uint32_t load_relaxed(const uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
@@ -74,93 +79,89 @@ __device__ __forceinline__ uint32_t rank_of(const uint32_t *mask,
   return prefix[w] + static_cast<uint32_t>(__popc(mask[w] & ((1u << (x & 31u)) - 1u)));
 }
 
-__device__ __forceinline__ uint32_t live_count(const uint32_t *counter,
-                                               uint32_t capacity) {
-  const uint32_t n = *counter;
-  return n > capacity ? 0u : n; // an overfull frame is refused, not labelled
+// An overfull frame is refused, not labelled.
+__device__ __forceinline__ uint32_t live_count(const FrameDev &f) {
+  const uint32_t n = f.counter[0];
+  return n > f.capacity ? 0u : n;
 }
 
+#define FOR_EACH_PIXEL(i, n)                                                   \
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n);            \
+       i += gridDim.x * blockDim.x)
+
 // parent[i] = i - 1 when the left neighbour is set, else i.
-__global__ __launch_bounds__(256) void link_left(
-    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
-    uint32_t capacity, uint32_t width, uint32_t *__restrict__ parent) {
-  const uint32_t n = live_count(counter, capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    const uint32_t idx = list[i].index;
-    const bool left = i > 0 && idx % width != 0 && list[i - 1].index == idx - 1;
-    parent[i] = left ? i - 1 : i;
+__global__ __launch_bounds__(256) void link_left(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t n = live_count(f);
+  FOR_EACH_PIXEL(i, n) {
+    const uint32_t idx = f.ordered[i].index;
+    const bool left =
+        i > 0 && idx % f.width != 0 && f.ordered[i - 1].index == idx - 1;
+    f.parent[i] = left ? i - 1 : i;
   }
 }
 
 // Union with the pixel above.
-__global__ __launch_bounds__(256) void link_up(
-    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
-    uint32_t capacity, uint32_t width, const uint32_t *__restrict__ mask,
-    const uint32_t *__restrict__ prefix, uint32_t wpr, uint32_t *parent) {
-  const uint32_t n = live_count(counter, capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    const uint32_t idx = list[i].index;
-    const uint32_t y = idx / width, x = idx - y * width;
-    if (y > 0 && bit_set(mask, wpr, x, y - 1))
-      unite(parent, i, rank_of(mask, prefix, wpr, x, y - 1));
+__global__ __launch_bounds__(256) void link_up(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t n = live_count(f);
+  FOR_EACH_PIXEL(i, n) {
+    const uint32_t idx = f.ordered[i].index;
+    const uint32_t y = idx / f.width, x = idx - y * f.width;
+    if (y > 0 && bit_set(f.mask, f.wpr, x, y - 1))
+      unite(f.parent, i, rank_of(f.mask, f.prefix, f.wpr, x, y - 1));
   }
 }
 
-// Flatten every tree to its root; mark the roots in a bitmask of ranks.
-// A wave covers 64 consecutive ranks from a multiple of 64, so its ballot is
-// two whole words.
-__global__ __launch_bounds__(256) void flatten(
-    const uint32_t *__restrict__ counter, uint32_t capacity, uint32_t *parent,
-    uint32_t *__restrict__ roots) {
-  const uint32_t n = live_count(counter, capacity);
+// Flatten every tree to its root; mark the roots in a bitmask of ranks.  A
+// wave covers 64 consecutive ranks from a multiple of 64, so its ballot is
+// two whole words, and the words up to the live count are all written.
+__global__ __launch_bounds__(256) void flatten(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t n = live_count(f);
   const uint32_t span = (n + 63u) & ~63u;
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < span;
-       i += gridDim.x * blockDim.x) {
+  FOR_EACH_PIXEL(i, span) {
     bool root = false;
     if (i < n) {
-      uint32_t r = parent[i];
-      while (parent[r] != r)
-        r = parent[r];
-      parent[i] = r;
+      uint32_t r = load_relaxed(&f.parent[i]);
+      for (;;) {
+        const uint32_t up = load_relaxed(&f.parent[r]);
+        if (up == r)
+          break;
+        r = up;
+      }
+      store_relaxed(&f.parent[i], r);
       root = r == i;
     }
     const uint64_t b = __ballot(root);
     const uint32_t lane = sfd::lane_id();
     if (lane == 0)
-      roots[i / 32] = static_cast<uint32_t>(b);
+      f.roots[i / 32] = static_cast<uint32_t>(b);
     else if (lane == 32)
-      roots[i / 32] = static_cast<uint32_t>(b >> 32);
+      f.roots[i / 32] = static_cast<uint32_t>(b >> 32);
   }
 }
 
-__device__ __forceinline__ uint32_t component_of(const uint32_t *roots,
-                                                 const uint32_t *root_prefix,
-                                                 uint32_t r) {
-  return root_prefix[r / 32] +
-         static_cast<uint32_t>(__popc(roots[r / 32] & ((1u << (r & 31u)) - 1u)));
+__device__ __forceinline__ uint32_t component_of(const FrameDev &f, uint32_t r) {
+  return f.root_prefix[r / 32] +
+         static_cast<uint32_t>(__popc(f.roots[r / 32] & ((1u << (r & 31u)) - 1u)));
 }
 
 // Component of every pixel; each root opens its component's record.
 __global__ __launch_bounds__(256) void number_components(
-    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
-    uint32_t capacity, const uint32_t *__restrict__ parent,
-    const uint32_t *__restrict__ roots, const uint32_t *__restrict__ root_prefix,
-    uint32_t *__restrict__ comp_of, sf_component *__restrict__ comps,
-    uint32_t *__restrict__ ncomp) {
-  const uint32_t n = live_count(counter, capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    const uint32_t r = parent[i];
-    const uint32_t c = component_of(roots, root_prefix, r);
-    comp_of[i] = c;
+    const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t n = live_count(f);
+  FOR_EACH_PIXEL(i, n) {
+    const uint32_t r = f.parent[i];
+    const uint32_t c = component_of(f, r);
+    f.comp_of[i] = c;
     if (r == i) {
       sf_component z{};
-      z.first = list[i].index;
+      z.first = f.ordered[i].index;
       z.x0 = z.y0 = 0xFFFFFFFFu;
-      comps[c] = z;
-      atomicAdd(ncomp, 1u);
+      f.comps[c] = z;
+      atomicAdd(&f.counter[2], 1u);
     }
   }
 }
@@ -176,16 +177,13 @@ __device__ __forceinline__ void add128(uint64_t *pair, uint64_t lo, uint64_t hi)
 }
 
 // Each pixel's exact moments into its component.
-__global__ __launch_bounds__(256) void accumulate(
-    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
-    uint32_t capacity, uint32_t width, const uint32_t *__restrict__ comp_of,
-    sf_component *comps) {
-  const uint32_t n = live_count(counter, capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    const sf_signal_pixel p = list[i];
-    const uint32_t y = p.index / width, x = p.index - y * width;
-    sf_component *c = &comps[comp_of[i]];
+__global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t n = live_count(f);
+  FOR_EACH_PIXEL(i, n) {
+    const sf_signal_pixel p = f.ordered[i];
+    const uint32_t y = p.index / f.width, x = p.index - y * f.width;
+    sf_component *c = &f.comps[f.comp_of[i]];
     const uint64_t v = p.value;
     const uint64_t ax = 2ull * x + 1ull, ay = 2ull * y + 1ull;
     atomicAdd(&c->n, 1u);
@@ -208,30 +206,28 @@ __global__ __launch_bounds__(256) void accumulate(
 
 // Pairs (component here, component on the previous frame) for every pixel
 // set in both frames; a pixel repeating its left neighbour's pair is skipped.
-__global__ __launch_bounds__(256) void link_frames(
-    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
-    uint32_t capacity, uint32_t width, const uint32_t *__restrict__ comp_of,
-    const uint32_t *__restrict__ prev_mask, const uint32_t *__restrict__ prev_prefix,
-    const uint32_t *__restrict__ prev_comp_of, uint32_t wpr,
-    uint32_t *__restrict__ links, uint32_t *__restrict__ nlinks,
-    uint32_t link_capacity) {
-  const uint32_t n = live_count(counter, capacity);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
-       i += gridDim.x * blockDim.x) {
-    const uint32_t idx = list[i].index;
-    const uint32_t y = idx / width, x = idx - y * width;
-    if (!bit_set(prev_mask, wpr, x, y))
+__global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  if (f.prev_mask == nullptr)
+    return;
+  const uint32_t n = live_count(f);
+  FOR_EACH_PIXEL(i, n) {
+    const uint32_t idx = f.ordered[i].index;
+    const uint32_t y = idx / f.width, x = idx - y * f.width;
+    if (!bit_set(f.prev_mask, f.wpr, x, y))
       continue;
-    const uint32_t here = comp_of[i];
-    const uint32_t there = prev_comp_of[rank_of(prev_mask, prev_prefix, wpr, x, y)];
-    if (x > 0 && i > 0 && list[i - 1].index == idx - 1 && comp_of[i - 1] == here &&
-        bit_set(prev_mask, wpr, x - 1, y) &&
-        prev_comp_of[rank_of(prev_mask, prev_prefix, wpr, x - 1, y)] == there)
+    const uint32_t here = f.comp_of[i];
+    const uint32_t there =
+        f.prev_comp_of[rank_of(f.prev_mask, f.prev_prefix, f.wpr, x, y)];
+    if (x > 0 && i > 0 && f.ordered[i - 1].index == idx - 1 &&
+        f.comp_of[i - 1] == here && bit_set(f.prev_mask, f.wpr, x - 1, y) &&
+        f.prev_comp_of[rank_of(f.prev_mask, f.prev_prefix, f.wpr, x - 1, y)] ==
+            there)
       continue;
-    const uint32_t at = atomicAdd(nlinks, 1u);
-    if (at < link_capacity) {
-      links[2 * at] = here;
-      links[2 * at + 1] = there;
+    const uint32_t at = atomicAdd(&f.counter[3], 1u);
+    if (at < f.link_capacity) {
+      f.links[2 * at] = here;
+      f.links[2 * at + 1] = there;
     }
   }
 }
@@ -240,28 +236,21 @@ __global__ __launch_bounds__(256) void link_frames(
 
 namespace sfi {
 
-void launch_label(const LabelArgs &a, hipStream_t stream) {
-  const uint32_t blocks = 512;
-  launch_fill(a.ncomp, 1, 0u, stream);
-  link_left<<<blocks, 256, 0, stream>>>(a.list, a.counter, a.capacity, a.width,
-                                        a.parent);
-  link_up<<<blocks, 256, 0, stream>>>(a.list, a.counter, a.capacity, a.width,
-                                      a.mask, a.prefix, a.wpr, a.parent);
-  flatten<<<blocks, 256, 0, stream>>>(a.counter, a.capacity, a.parent, a.roots);
-  launch_popcount_scan(a.roots, (static_cast<size_t>(a.capacity) + 63) / 32,
-                       a.counter, a.capacity, a.root_prefix, a.scan_tmp, stream);
-  number_components<<<blocks, 256, 0, stream>>>(a.list, a.counter, a.capacity,
-                                                a.parent, a.roots, a.root_prefix,
-                                                a.comp_of, a.comps, a.ncomp);
-  accumulate<<<blocks, 256, 0, stream>>>(a.list, a.counter, a.capacity, a.width,
-                                         a.comp_of, a.comps);
-  if (a.prev_mask) {
-    launch_fill(a.nlinks, 1, 0u, stream);
-    link_frames<<<blocks, 256, 0, stream>>>(
-        a.list, a.counter, a.capacity, a.width, a.comp_of, a.prev_mask,
-        a.prev_prefix, a.prev_comp_of, a.wpr, a.links, a.nlinks,
-        a.link_capacity);
-  }
+// frames_device: the frames' descriptors; root_segs_device: their root-bit
+// scans; both in device memory, nframes long.  counter[2] and counter[3] of
+// every frame are cleared by the caller.
+void launch_label(const FrameDev *frames_device, const ScanSeg *root_segs_device,
+                  uint32_t nframes, size_t max_root_words, hipStream_t stream) {
+  if (nframes == 0)
+    return;
+  const dim3 grid(256, nframes);
+  link_left<<<grid, 256, 0, stream>>>(frames_device);
+  link_up<<<grid, 256, 0, stream>>>(frames_device);
+  flatten<<<grid, 256, 0, stream>>>(frames_device);
+  launch_popcount_scan(root_segs_device, nframes, max_root_words, stream);
+  number_components<<<grid, 256, 0, stream>>>(frames_device);
+  accumulate<<<grid, 256, 0, stream>>>(frames_device);
+  link_frames<<<grid, 256, 0, stream>>>(frames_device);
   check(hipGetLastError(), "launching the labelling");
 }
 

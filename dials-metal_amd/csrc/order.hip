@@ -1,4 +1,5 @@
-// Putting a frame's signal pixels into ascending index order on the device.
+// Putting each frame's signal pixels into ascending index order on the
+// device, for a batch of frames at once.
 //
 // The reference sorts the device's unordered emit on the host
 // (signal_order::by_index, src/spots/signal_order.cc:19-72).  Here the
@@ -9,15 +10,19 @@
 //     base[word] + popcount(word & bits below it).
 // O(n) in the frame's words plus the list, no comparisons, and the order is a
 // property of the frame alone, not of the order the waves finished in.
+//
+// Every kernel here takes an array of per-frame descriptors and works on
+// frame blockIdx.y, so a batch costs the same few launches as one frame; the
+// scans are segmented by frame, each frame's prefix starting at zero.
 #include "sf_device.hh"
 #include "sf_internal.hh"
 
 namespace {
 
-// The scan: blocks of 256 threads, 16 words a thread, 4096 words a block.
+using sfi::FrameDev;
+using sfi::kScanPerBlock;
 constexpr int kScanThreads = 256;
-constexpr int kPerThread = 16;
-constexpr int kPerBlock = kScanThreads * kPerThread;
+constexpr int kPerThread = kScanPerBlock / kScanThreads;
 
 // Inclusive scan of one value across the block; returns this thread's
 // inclusive value and the block total through *total.
@@ -45,106 +50,123 @@ __device__ __forceinline__ uint32_t block_inclusive(uint32_t v, uint32_t *total)
   return v + before;
 }
 
-// Words in use: all of them, or with a live bit count those that hold it
-// (none for a count above capacity, which the caller refuses anyway).
-__device__ __forceinline__ size_t live_words(size_t words, const uint32_t *live,
-                                             uint32_t capacity) {
-  if (live == nullptr)
-    return words;
-  const uint32_t n = *live;
-  const size_t w = n > capacity ? 0 : (static_cast<size_t>(n) + 31) / 32;
-  return w < words ? w : words;
+// The words of frame blockIdx.y's bit array that are in use: all of them,
+// or with a live count only those that hold it (none for a count above the
+// capacity, which the caller refuses anyway).
+__device__ __forceinline__ size_t live_words(const sfi::ScanSeg &s) {
+  if (s.live == nullptr)
+    return s.words;
+  const uint32_t n = *s.live;
+  const size_t w = n > s.capacity ? 0 : (static_cast<size_t>(n) + 31) / 32;
+  return w < s.words ? w : s.words;
 }
 
-// Per-block totals of popcounts.
+// Per-block totals of popcounts; block (x, frame).
 __global__ __launch_bounds__(kScanThreads) void popcount_totals(
-    const uint32_t *__restrict__ mask, size_t words, const uint32_t *live,
-    uint32_t capacity, uint32_t *__restrict__ totals) {
-  words = live_words(words, live, capacity);
-  const size_t base = static_cast<size_t>(blockIdx.x) * kPerBlock +
+    const sfi::ScanSeg *__restrict__ segs) {
+  const sfi::ScanSeg s = segs[blockIdx.y];
+  const size_t words = live_words(s);
+  if (blockIdx.x * static_cast<size_t>(kScanPerBlock) >= s.words)
+    return;
+  const size_t base = static_cast<size_t>(blockIdx.x) * kScanPerBlock +
                       static_cast<size_t>(threadIdx.x) * kPerThread;
   uint32_t c = 0;
 #pragma unroll
   for (int i = 0; i < kPerThread; i++)
     if (base + i < words)
-      c += static_cast<uint32_t>(__popc(mask[base + i]));
+      c += static_cast<uint32_t>(__popc(s.bits[base + i]));
   uint32_t total;
   (void)block_inclusive(c, &total);
   if (threadIdx.x == 0)
-    totals[blockIdx.x] = total;
+    s.totals[blockIdx.x] = total;
 }
 
-// Exclusive scan of the block totals in place, by one block.
-__global__ __launch_bounds__(kScanThreads) void scan_totals(uint32_t *totals,
-                                                            uint32_t n) {
+// Exclusive scan of one frame's block totals, in place; one block a frame.
+__global__ __launch_bounds__(kScanThreads) void scan_totals(
+    const sfi::ScanSeg *__restrict__ segs) {
+  const sfi::ScanSeg s = segs[blockIdx.y];
+  const uint32_t n =
+      static_cast<uint32_t>((s.words + kScanPerBlock - 1) / kScanPerBlock);
   uint32_t carry = 0;
   for (uint32_t start = 0; start < n; start += kScanThreads) {
     const uint32_t i = start + threadIdx.x;
-    const uint32_t v = i < n ? totals[i] : 0u;
+    const uint32_t v = i < n ? s.totals[i] : 0u;
     uint32_t total;
     const uint32_t inc = block_inclusive(v, &total);
     if (i < n)
-      totals[i] = carry + inc - v;
+      s.totals[i] = carry + inc - v;
     carry += total;
   }
 }
 
-// Exclusive prefix of popcounts per word.
+// Exclusive prefix of popcounts per word, frame-local.
 __global__ __launch_bounds__(kScanThreads) void popcount_prefix(
-    const uint32_t *__restrict__ mask, size_t words, const uint32_t *live,
-    uint32_t capacity, const uint32_t *__restrict__ totals,
-    uint32_t *__restrict__ prefix) {
-  words = live_words(words, live, capacity);
-  const size_t base = static_cast<size_t>(blockIdx.x) * kPerBlock +
+    const sfi::ScanSeg *__restrict__ segs) {
+  const sfi::ScanSeg s = segs[blockIdx.y];
+  if (blockIdx.x * static_cast<size_t>(kScanPerBlock) >= s.words)
+    return;
+  const size_t words = live_words(s);
+  const size_t base = static_cast<size_t>(blockIdx.x) * kScanPerBlock +
                       static_cast<size_t>(threadIdx.x) * kPerThread;
   uint32_t c[kPerThread];
   uint32_t sum = 0;
 #pragma unroll
   for (int i = 0; i < kPerThread; i++) {
-    c[i] = base + i < words ? static_cast<uint32_t>(__popc(mask[base + i])) : 0u;
+    c[i] = base + i < words ? static_cast<uint32_t>(__popc(s.bits[base + i])) : 0u;
     sum += c[i];
   }
   uint32_t total;
-  uint32_t run = block_inclusive(sum, &total) - sum + totals[blockIdx.x];
+  uint32_t run = block_inclusive(sum, &total) - sum + s.totals[blockIdx.x];
 #pragma unroll
   for (int i = 0; i < kPerThread; i++) {
     if (base + i < words)
-      prefix[base + i] = run;
+      s.prefix[base + i] = run;
     run += c[i];
   }
 }
 
 __global__ __launch_bounds__(256) void scatter_by_rank(
-    const sf_signal_pixel *__restrict__ in, sf_signal_pixel *__restrict__ out,
-    const uint32_t *__restrict__ counter, uint32_t capacity,
-    const uint32_t *__restrict__ mask, const uint32_t *__restrict__ base,
-    uint32_t width, uint32_t wpr) {
-  const uint32_t n = *counter;
-  if (n > capacity) // the list is incomplete; the caller reports -2
+    const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t n = f.counter[0];
+  if (n > f.capacity) // the list is incomplete; the caller reports -2
     return;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    const sf_signal_pixel rec = in[i];
-    const uint32_t y = rec.index / width;
-    const uint32_t x = rec.index - y * width;
-    const size_t word = static_cast<size_t>(y) * wpr + x / 32;
-    const uint32_t below = mask[word] & ((1u << (x & 31u)) - 1u);
-    out[base[word] + static_cast<uint32_t>(__popc(below))] = rec;
+    const sf_signal_pixel rec = f.unordered[i];
+    const uint32_t y = rec.index / f.width;
+    const uint32_t x = rec.index - y * f.width;
+    const size_t word = static_cast<size_t>(y) * f.wpr + x / 32;
+    const uint32_t below = f.mask[word] & ((1u << (x & 31u)) - 1u);
+    f.ordered[f.prefix[word] + static_cast<uint32_t>(__popc(below))] = rec;
   }
 }
 
-__global__ __launch_bounds__(256) void set_bits(
-    const sf_signal_pixel *__restrict__ list, const uint32_t *__restrict__ counter,
-    uint32_t capacity, uint32_t width, uint32_t *__restrict__ mask, uint32_t wpr) {
-  const uint32_t n = *counter;
-  if (n > capacity)
+__global__ __launch_bounds__(256) void set_bits(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t n = f.counter[0];
+  if (n > f.capacity)
     return;
   for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
        i += gridDim.x * blockDim.x) {
-    const uint32_t idx = list[i].index;
-    const uint32_t y = idx / width, x = idx - y * width;
-    atomicOr(&mask[static_cast<size_t>(y) * wpr + x / 32], 1u << (x & 31u));
+    const uint32_t idx = f.ordered[i].index;
+    const uint32_t y = idx / f.width, x = idx - y * f.width;
+    atomicOr(&f.mask[static_cast<size_t>(y) * f.wpr + x / 32], 1u << (x & 31u));
   }
+}
+
+__global__ __launch_bounds__(256) void clear_frames(
+    const FrameDev *__restrict__ frames, int masks, int first) {
+  const FrameDev f = frames[blockIdx.y];
+  if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) >= first &&
+      threadIdx.x < 4)
+    f.counter[threadIdx.x] = 0;
+  if (!masks)
+    return;
+  const size_t words = static_cast<size_t>(f.height) * f.wpr;
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+       i < words; i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    f.mask[i] = 0;
 }
 
 __global__ __launch_bounds__(256) void fill_words(uint32_t *p, size_t n,
@@ -158,47 +180,46 @@ __global__ __launch_bounds__(256) void fill_words(uint32_t *p, size_t n,
 
 namespace sfi {
 
-size_t scan_tmp_bytes(size_t words) {
-  return ((words + kPerBlock - 1) / kPerBlock + 1) * sizeof(uint32_t);
+size_t scan_tmp_words(size_t words) {
+  return (words + kScanPerBlock - 1) / kScanPerBlock + 1;
 }
 
-size_t order_scan_bytes(size_t words) { return scan_tmp_bytes(words); }
-
-void launch_popcount_scan(const uint32_t *bits, size_t words,
-                          const uint32_t *live, uint32_t capacity,
-                          uint32_t *prefix, void *tmp, hipStream_t stream) {
+void launch_popcount_scan(const ScanSeg *segs_device, uint32_t nframes,
+                          size_t max_words, hipStream_t stream) {
   const uint32_t blocks =
-      static_cast<uint32_t>((words + kPerBlock - 1) / kPerBlock);
-  if (blocks == 0)
+      static_cast<uint32_t>((max_words + kScanPerBlock - 1) / kScanPerBlock);
+  if (blocks == 0 || nframes == 0)
     return;
-  uint32_t *totals = static_cast<uint32_t *>(tmp);
-  popcount_totals<<<blocks, kScanThreads, 0, stream>>>(bits, words, live,
-                                                       capacity, totals);
-  scan_totals<<<1, kScanThreads, 0, stream>>>(totals, blocks);
-  popcount_prefix<<<blocks, kScanThreads, 0, stream>>>(bits, words, live,
-                                                       capacity, totals, prefix);
+  popcount_totals<<<dim3(blocks, nframes), kScanThreads, 0, stream>>>(segs_device);
+  scan_totals<<<dim3(1, nframes), kScanThreads, 0, stream>>>(segs_device);
+  popcount_prefix<<<dim3(blocks, nframes), kScanThreads, 0, stream>>>(segs_device);
   check(hipGetLastError(), "launching a popcount scan");
 }
 
-void launch_order(const FrameArgs &a, const sf_signal_pixel *unordered,
-                  sf_signal_pixel *ordered, uint32_t *prefix, void *scan_tmp,
-                  size_t scan_tmp_bytes, hipStream_t stream) {
-  const size_t words = static_cast<size_t>(a.height) * a.wpr;
-  if (scan_tmp_bytes < order_scan_bytes(words))
-    throw device_error("the rank scan's scratch is too small");
-  launch_popcount_scan(a.mask, words, nullptr, 0, prefix, scan_tmp, stream);
+void launch_clear(const FrameDev *frames_device, uint32_t nframes, bool masks,
+                  int first, hipStream_t stream) {
+  if (nframes == 0)
+    return;
+  clear_frames<<<dim3(masks ? 256 : 1, nframes), 256, 0, stream>>>(
+      frames_device, masks ? 1 : 0, first);
+  check(hipGetLastError(), "launching the frame clear");
+}
+
+void launch_scatter(const FrameDev *frames_device, uint32_t nframes,
+                    hipStream_t stream) {
+  if (nframes == 0)
+    return;
   // Enough blocks to cover a dense frame in a few strides; the count itself
   // is only known on the device.
-  scatter_by_rank<<<1024, 256, 0, stream>>>(unordered, ordered, a.counter,
-                                            a.capacity, a.mask, prefix,
-                                            a.width, a.wpr);
+  scatter_by_rank<<<dim3(256, nframes), 256, 0, stream>>>(frames_device);
   check(hipGetLastError(), "launching the rank scatter");
 }
 
-void launch_set_bits(const sf_signal_pixel *list, const uint32_t *counter,
-                     uint32_t capacity, uint32_t width, uint32_t *mask,
-                     uint32_t wpr, hipStream_t stream) {
-  set_bits<<<512, 256, 0, stream>>>(list, counter, capacity, width, mask, wpr);
+void launch_set_bits(const FrameDev *frames_device, uint32_t nframes,
+                     hipStream_t stream) {
+  if (nframes == 0)
+    return;
+  set_bits<<<dim3(256, nframes), 256, 0, stream>>>(frames_device);
   check(hipGetLastError(), "launching set_bits");
 }
 

@@ -27,6 +27,39 @@ inline void check(hipError_t e, const char *what) {
 
 void set_last_error(const std::string &message);
 
+// The fused kernel's launch shape (sf_tuning in the public header).
+using Tuning = sf_tuning;
+Tuning tuning();
+
+// A grow-only device allocation.  Growing waits for the stream first: work in
+// flight may still be reading the old buffer (dext_cuda.cu:506-525).
+struct DeviceBuffer {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  DeviceBuffer() = default;
+  DeviceBuffer(const DeviceBuffer &) = delete;
+  DeviceBuffer &operator=(const DeviceBuffer &) = delete;
+  ~DeviceBuffer() {
+    if (ptr)
+      (void)hipFree(ptr);
+  }
+  // True when the buffer moved, which invalidates any graph holding it.
+  bool ensure(size_t want, hipStream_t stream) {
+    if (want <= bytes)
+      return false;
+    if (ptr) {
+      check(hipStreamSynchronize(stream), "waiting before a buffer grows");
+      check(hipFree(ptr), "hipFree");
+      ptr = nullptr;
+      bytes = 0;
+    }
+    check(hipMalloc(&ptr, want), "hipMalloc");
+    bytes = want;
+    return true;
+  }
+  template <typename P> P *as() const { return static_cast<P *>(ptr); }
+};
+
 // Words per row of the strong-pixel bitmask: one bit per pixel, rows padded
 // to whole 32-bit words, bit x % 32 of word y * wpr + x / 32.
 inline uint32_t mask_words_per_row(uint32_t width) { return (width + 31) / 32; }
@@ -51,57 +84,90 @@ struct FrameArgs {
 void launch_direct(const FrameArgs &a, uint8_t *scr0, uint8_t *scr1,
                    hipStream_t stream);
 
-// The fused single-kernel threshold.
-void launch_fused(const FrameArgs &a, hipStream_t stream);
+// One frame as the fused kernel sees it; a batch is an array of these in
+// device memory, with item_base the frame's first work item.
+struct FusedParams {
+  const void *frame;
+  uint32_t height, width;
+  sf_signal_pixel *out;
+  uint32_t *counter;
+  uint32_t capacity;
+  uint32_t *mask;
+  uint32_t wpr;
+  const void *table;
+  uint32_t strips, segments, seg_rows;
+  uint32_t *overflow;
+  uint32_t bits;
+  uint32_t item_base;
+};
 
-// Ordering of an unordered list by the bitmask's ranks: prefix is
-// height * wpr + 1 words of scratch; `scan_tmp` is grown as needed.
-size_t order_scan_bytes(size_t words);
-void launch_order(const FrameArgs &a, const sf_signal_pixel *unordered,
-                  sf_signal_pixel *ordered, uint32_t *prefix, void *scan_tmp,
-                  size_t scan_tmp_bytes, hipStream_t stream);
+// A frame's cut into work items (strips x segments) under a tuning.
+struct FusedPlan {
+  uint32_t strips = 0, segments = 0, seg_rows = 0;
+};
+uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
+                    const Tuning &t);
+FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,
+                      uint32_t item_base);
+// One persistent launch over every item of `nframes` frames (descriptors in
+// device memory), mixed sizes and pixel types; each frame's mask and
+// overflow word cleared beforehand.
+void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
+                        uint32_t total_items, const float *tf, const double *td,
+                        hipStream_t stream);
 
 // Clears `n` 32-bit words with a kernel, which unlike a memset node stays
 // correct when the sequence is captured into a graph and replayed.
 void launch_fill(uint32_t *p, size_t n, uint32_t value, hipStream_t stream);
 
-// Exclusive prefix of per-word popcounts over `words` words.  With `live`
-// (a device count of bits in use, refused as zero above `capacity`), words
-// past the live bits are read as zero.  `tmp` needs scan_tmp_bytes(words).
-size_t scan_tmp_bytes(size_t words);
-void launch_popcount_scan(const uint32_t *bits, size_t words,
-                          const uint32_t *live, uint32_t capacity,
-                          uint32_t *prefix, void *tmp, hipStream_t stream);
-
-// One frame's labelling: the ordered list and its bitmask in, components out.
-struct LabelArgs {
-  const sf_signal_pixel *list; // ordered, `*counter` entries
-  const uint32_t *counter;
+// One frame's bit array to scan: the exclusive prefix of per-word popcounts
+// (frame-local, from zero).  With `live` (a device count of bits in use,
+// refused as zero above `capacity`), words past the live bits read as zero.
+// `totals` holds scan_tmp_words(words) words.
+constexpr int kScanPerBlock = 4096; // words per scan block
+struct ScanSeg {
+  const uint32_t *bits;
+  uint32_t *prefix;
+  uint32_t *totals;
+  size_t words;
+  const uint32_t *live;
   uint32_t capacity;
-  uint32_t width;
-  const uint32_t *mask, *prefix; // the frame's bitmask and word prefix
-  uint32_t wpr;
-  uint32_t *parent;      // capacity
-  uint32_t *roots;       // capacity / 32 + 2 words
-  uint32_t *root_prefix; // same
-  void *scan_tmp;        // scan_tmp_bytes(capacity / 32 + 2)
-  uint32_t *comp_of;     // capacity
-  sf_component *comps;   // capacity
-  uint32_t *ncomp;       // device word
-  // The previous frame of the same shape, for three-dimensional links, or
-  // null.
+};
+size_t scan_tmp_words(size_t words);
+// One scan per frame, all frames in three launches; descriptors on device.
+void launch_popcount_scan(const ScanSeg *segs_device, uint32_t nframes,
+                          size_t max_words, hipStream_t stream);
+
+// One frame's device state after its threshold, for ordering and labelling.
+struct FrameDev {
+  sf_signal_pixel *unordered; // the threshold's emit
+  sf_signal_pixel *ordered;   // ascending by index
+  uint32_t *counter;          // [0] pixels [1] overflow [2] components [3] links
+  uint32_t capacity;
+  uint32_t width, height, wpr;
+  uint32_t *mask, *prefix;    // strong-pixel bitmask and its word prefix
+  // labelling
+  uint32_t *parent, *roots, *root_prefix, *comp_of;
+  sf_component *comps;
+  // the previous frame of the same shape, for three-dimensional links
   const uint32_t *prev_mask, *prev_prefix, *prev_comp_of;
-  uint32_t *links;       // 2 * link_capacity words
-  uint32_t *nlinks;      // device word
+  uint32_t *links;            // 2 * link_capacity words
   uint32_t link_capacity;
 };
 
-void launch_label(const LabelArgs &a, hipStream_t stream);
-
-// Sets the bitmask bit of every pixel of an ordered list of `*counter`
-// entries (at most `capacity`); the mask starts clear.
-void launch_set_bits(const sf_signal_pixel *list, const uint32_t *counter,
-                     uint32_t capacity, uint32_t width, uint32_t *mask,
-                     uint32_t wpr, hipStream_t stream);
+// Clears each frame's counters from `first` to the fourth and, with `masks`,
+// its bitmask.
+void launch_clear(const FrameDev *frames_device, uint32_t nframes, bool masks,
+                  int first, hipStream_t stream);
+// Places every frame's unordered emit at its rank (blockIdx.y = frame).
+void launch_scatter(const FrameDev *frames_device, uint32_t nframes,
+                    hipStream_t stream);
+// Sets the bitmask bit of every pixel of each frame's ordered list (a list
+// loaded from the host); the masks start clear.
+void launch_set_bits(const FrameDev *frames_device, uint32_t nframes,
+                     hipStream_t stream);
+// Labels every frame: components, moments, and links where prev_mask is set.
+void launch_label(const FrameDev *frames_device, const ScanSeg *root_segs_device,
+                  uint32_t nframes, size_t max_root_words, hipStream_t stream);
 
 } // namespace sfi

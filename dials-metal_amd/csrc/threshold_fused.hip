@@ -46,23 +46,12 @@ constexpr int kStrip = kQ * kLanes;     // 256 columns per wave
 constexpr int kHalo = 12;               // columns of halo each side
 constexpr int kOut = kStrip - 2 * kHalo; // 232 output columns per strip
 constexpr int kRing = 17;               // raw rows held: entering .. 16 above
-constexpr int kWaves = 2;               // independent waves per block
+constexpr uint32_t kStage = 128;        // signal pixels staged per wave
 constexpr uint32_t kFast = 1u << 18;    // fast-path bound on valid pixels
 constexpr uint32_t kCount0 = 1u << 26;  // count unit in stage 0's word
 constexpr uint32_t kCount2 = 1u << 25;  // count unit in stage 2's word
 
-struct FusedParams {
-  const void *frame;
-  uint32_t height, width;
-  sf_signal_pixel *out;
-  uint32_t *counter;
-  uint32_t capacity;
-  uint32_t *mask;
-  uint32_t wpr;
-  const void *table;
-  uint32_t strips, segments, seg_rows;
-  uint32_t *overflow;
-};
+using sfi::FusedParams;
 
 // Lane shifts across the whole wave.  DPP wave_shr:1 gives lane i the value of
 // lane i-1 (lane 0 gets 0); wave_shl:1 gives lane i the value of lane i+1
@@ -161,21 +150,19 @@ __device__ __forceinline__ bool stage0_test<uint16_t>(uint32_t word,
   return lhs > rhs;
 }
 
+// What one wave keeps in LDS: the ring of raw rows and the signal staging.
+struct WaveLds {
+  uint4 ring[kRing][kLanes];
+  sf_signal_pixel stage[kStage];
+};
+
+// One wave's sweep of one strip segment of one frame: `item` numbers the
+// frame's segments strip-fastest.
 template <typename T>
-__global__ __launch_bounds__(kWaves *kLanes) void fused_kernel(FusedParams a) {
+__device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
+                      WaveLds &w, const typename traits<T>::R *table) {
   using R = typename traits<T>::R;
   using Wt = typename traits<T>::W;
-  __shared__ uint4 ring_all[kWaves][kRing][kLanes];
-  __shared__ R table[128];
-  for (int i = threadIdx.x; i < 128; i += blockDim.x)
-    table[i] = static_cast<const R *>(a.table)[i];
-  __syncthreads();
-
-  const int wave = threadIdx.x / kLanes;
-  const uint32_t lane = threadIdx.x % kLanes;
-  const uint32_t item = blockIdx.x * kWaves + wave;
-  if (item >= a.strips * a.segments)
-    return;
   const int height = static_cast<int>(a.height);
   const int width = static_cast<int>(a.width);
   const int strip = static_cast<int>(item % a.strips);
@@ -185,8 +172,44 @@ __global__ __launch_bounds__(kWaves *kLanes) void fused_kernel(FusedParams a) {
   const int y0 = seg * static_cast<int>(a.seg_rows);
   const int y1 = min(height, y0 + static_cast<int>(a.seg_rows));
   const T *frame = static_cast<const T *>(a.frame);
-  uint4(*ring)[kLanes] = ring_all[wave];
+  uint4(*ring)[kLanes] = w.ring;
   const uint32_t masked = traits<T>::masked;
+
+  // Signal pixels wait in LDS and go out kStage at a time, one atomic on the
+  // frame's counter per flush: a returning atomic per row on that one word
+  // serialises every wave of the device behind it.
+  sf_signal_pixel *stage = w.stage;
+  uint32_t staged = 0; // wave-uniform
+  auto flush = [&]() {
+    if (staged == 0)
+      return;
+    uint32_t base = 0;
+    if (lane == 0)
+      base = atomicAdd(a.counter, staged);
+    base = __shfl(base, 0);
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+    for (uint32_t i = lane; i < staged; i += kLanes)
+      if (base + i < a.capacity)
+        a.out[base + i] = stage[i];
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    staged = 0;
+  };
+  auto put = [&](bool signal, const sf_signal_pixel &rec) {
+    const uint64_t b = __ballot(signal);
+    if (b == 0)
+      return;
+    const uint32_t n = static_cast<uint32_t>(__popcll(b));
+    if (staged + n > kStage)
+      flush();
+    if (signal)
+      stage[staged + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)))] =
+          rec;
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    staged += n;
+  };
+  bool saw_big = false;
 
   // Output columns of this lane inside the strip's middle and the frame.
   uint32_t in_out = 0;
@@ -223,8 +246,7 @@ __global__ __launch_bounds__(kWaves *kLanes) void fused_kernel(FusedParams a) {
 #pragma unroll
     for (int q = 0; q < kQ; q++)
       big |= cur[q] < masked && cur[q] >= kFast;
-    if (__ballot(big) != 0 && lane == 0)
-      atomicOr(a.overflow, 1u);
+    saw_big = saw_big || __ballot(big) != 0;
 
     ring[slot][lane] = make_uint4(cur[0], cur[1], cur[2], cur[3]);
     uint32_t mnib = 0;
@@ -357,39 +379,75 @@ __global__ __launch_bounds__(kWaves *kLanes) void fused_kernel(FusedParams a) {
               atomicOr(&a.mask[static_cast<size_t>(y2) * a.wpr + c / 32],
                        1u << (c & 31));
           }
-          emit_wave(signal, rec, a.out, a.counter, a.capacity);
+          put(signal, rec);
         }
       }
     }
     slot = slot + 1 == kRing ? 0 : slot + 1;
   }
+  flush();
+  if (saw_big && lane == 0)
+    atomicOr(a.overflow, 1u);
 }
 
-template <typename T>
-void run(const sfi::FrameArgs &f, uint32_t *overflow, hipStream_t stream) {
-  FusedParams p;
-  p.frame = f.frame;
-  p.height = f.height;
-  p.width = f.width;
-  p.out = f.out;
-  p.counter = f.counter;
-  p.capacity = f.capacity;
-  p.mask = f.mask;
-  p.wpr = f.wpr;
-  p.table = f.table;
-  p.strips = (f.width + kOut - 1) / kOut;
-  // Segments tall enough that the 20 halo rows cost little, short enough
-  // that a single frame still spreads over the whole device.
-  const uint32_t target_items = 2048;
-  uint32_t seg = (f.height * p.strips + target_items - 1) / target_items;
-  if (seg < 64)
-    seg = 64;
-  p.seg_rows = seg;
-  p.segments = (f.height + seg - 1) / seg;
-  p.overflow = overflow;
-  const uint32_t items = p.strips * p.segments;
-  const uint32_t blocks = (items + kWaves - 1) / kWaves;
-  fused_kernel<T><<<blocks, kWaves * kLanes, 0, stream>>>(p);
+// Every wave takes items in turn, grid-stride, over all the frames of a
+// batch (item_base ascending), any mix of sizes and pixel types in one
+// launch.  The grid is sized to stay resident, so the launch is persistent
+// and every wave leaves when the items run out.
+template <int Waves>
+__global__ __launch_bounds__(Waves *kLanes) void fused_kernel(
+    const FusedParams *__restrict__ frames, uint32_t nframes,
+    uint32_t total_items, const float *__restrict__ tf,
+    const double *__restrict__ td) {
+  __shared__ WaveLds lds[Waves];
+  __shared__ float table_f[128];
+  __shared__ double table_d[128];
+  for (int i = threadIdx.x; i < 128; i += blockDim.x) {
+    table_f[i] = tf[i];
+    table_d[i] = td[i];
+  }
+  __syncthreads();
+  const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
+  const uint32_t lane = threadIdx.x % kLanes;
+  for (uint32_t item = blockIdx.x * Waves + wave; item < total_items;
+       item += gridDim.x * Waves) {
+    uint32_t lo = 0, hi = nframes;
+    while (hi - lo > 1) {
+      const uint32_t mid = (lo + hi) / 2;
+      if (frames[mid].item_base <= item)
+        lo = mid;
+      else
+        hi = mid;
+    }
+    lo = __builtin_amdgcn_readfirstlane(lo);
+    const FusedParams p = frames[lo];
+    if (p.bits == 16)
+      sweep<uint16_t>(p, item - p.item_base, lane, lds[wave], table_f);
+    else
+      sweep<uint32_t>(p, item - p.item_base, lane, lds[wave], table_d);
+  }
+}
+
+void launch(const FusedParams *frames, uint32_t nframes, uint32_t total_items,
+            const float *tf, const double *td, int waves_per_block,
+            uint32_t max_waves, hipStream_t stream) {
+  if (total_items == 0)
+    return;
+  const uint32_t waves = total_items < max_waves ? total_items : max_waves;
+  switch (waves_per_block) {
+  case 1:
+    fused_kernel<1><<<waves, kLanes, 0, stream>>>(frames, nframes, total_items,
+                                                  tf, td);
+    break;
+  case 4:
+    fused_kernel<4><<<(waves + 3) / 4, 4 * kLanes, 0, stream>>>(
+        frames, nframes, total_items, tf, td);
+    break;
+  default:
+    fused_kernel<2><<<(waves + 1) / 2, 2 * kLanes, 0, stream>>>(
+        frames, nframes, total_items, tf, td);
+    break;
+  }
   sfi::check(hipGetLastError(), "launching the fused threshold");
 }
 
@@ -397,13 +455,59 @@ void run(const sfi::FrameArgs &f, uint32_t *overflow, hipStream_t stream) {
 
 namespace sfi {
 
-void launch_fused(const FrameArgs &a, hipStream_t stream) {
-  // The caller clears the bitmask (it is or-ed into bit by bit) and the
-  // overflow word before this launch.
-  if (a.bits == 16)
-    run<uint16_t>(a, a.overflow, stream);
-  else
-    run<uint32_t>(a, a.overflow, stream);
+uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
+                    const Tuning &t) {
+  p.strips = (width + kOut - 1) / kOut;
+  uint32_t seg = static_cast<uint32_t>(t.seg_rows);
+  if (seg == 0) {
+    // Segments tall enough that the 20 halo rows cost little, short enough
+    // that a single frame still spreads over the whole device.
+    const uint32_t target = static_cast<uint32_t>(t.target_waves);
+    seg = static_cast<uint32_t>(
+        (static_cast<uint64_t>(height) * p.strips + target - 1) / target);
+    if (seg < 16)
+      seg = 16;
+  }
+  p.seg_rows = seg;
+  p.segments = (height + seg - 1) / seg;
+  return p.strips * p.segments;
+}
+
+void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
+                        uint32_t total_items, const float *tf, const double *td,
+                        hipStream_t stream) {
+  const Tuning t = tuning();
+  // Resident waves: sized so that every CU holds several; the items are
+  // spread over them grid-stride.
+  int cus = 256;
+  int device = 0;
+  if (hipGetDevice(&device) == hipSuccess)
+    (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
+                                device);
+  const uint32_t max_waves = static_cast<uint32_t>(cus) * 16u;
+  launch(frames_device, nframes, total_items, tf, td, t.waves_per_block,
+         max_waves, stream);
+}
+
+FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,
+                      uint32_t item_base) {
+  FusedParams p;
+  p.frame = a.frame;
+  p.height = a.height;
+  p.width = a.width;
+  p.out = a.out;
+  p.counter = a.counter;
+  p.capacity = a.capacity;
+  p.mask = a.mask;
+  p.wpr = a.wpr;
+  p.table = a.table;
+  p.strips = plan.strips;
+  p.segments = plan.segments;
+  p.seg_rows = plan.seg_rows;
+  p.overflow = a.overflow;
+  p.bits = static_cast<uint32_t>(a.bits);
+  p.item_base = item_base;
+  return p;
 }
 
 } // namespace sfi

@@ -50,6 +50,20 @@ class Counts(ctypes.Structure):
     _fields_ = [(f, ctypes.c_uint64) for f in COUNTS_FIELDS]
 
 
+class FrameDesc(ctypes.Structure):
+    _fields_ = [("frame", ctypes.c_void_p), ("height", ctypes.c_uint32),
+                ("width", ctypes.c_uint32), ("bits", ctypes.c_int32)]
+
+
+class Tuning(ctypes.Structure):
+    _fields_ = [("seg_rows", ctypes.c_int), ("target_waves", ctypes.c_int),
+                ("waves_per_block", ctypes.c_int),
+                ("frames_per_launch", ctypes.c_int)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class SpotfinderError(RuntimeError):
     """A status other than success from the C ABI."""
 
@@ -106,6 +120,8 @@ def lib():
         "sf_copy_to_device": ([vp, vp, sz], ctypes.c_int),
         "sf_copy_to_host": ([vp, vp, sz], ctypes.c_int),
         "sf_device_synchronize": ([], ctypes.c_int),
+        "sf_get_tuning": ([ctypes.POINTER(Tuning)], None),
+        "sf_set_tuning": ([ctypes.POINTER(Tuning)], ctypes.c_int),
         "sf_set_kernel": ([ctypes.c_int], None),
         "sf_get_kernel": ([], ctypes.c_int),
         "sf_profile_stages": ([ctypes.c_int], None),
@@ -131,6 +147,23 @@ def lib():
                              ctypes.c_int),
         "sf_ws_fetch_labels": ([vp, vp, sz, vp, sz, vp, sz], ctypes.c_int),
         "sf_ws_load_signal": ([vp, vp, sz, u32, u32], ctypes.c_int),
+        "sf_batch_create": ([vp, ctypes.POINTER(vp)], ctypes.c_int),
+        "sf_batch_destroy": ([vp], None),
+        "sf_batch_stream": ([vp], vp),
+        "sf_batch_threshold": ([vp, ctypes.POINTER(FrameDesc), sz, ctypes.c_int],
+                               ctypes.c_int),
+        "sf_batch_sync": ([vp, ctypes.POINTER(sz), ctypes.POINTER(ctypes.c_int)],
+                          ctypes.c_int),
+        "sf_batch_fetch": ([vp, sz, vp, sz], ctypes.c_int),
+        "sf_batch_label": ([vp, ctypes.c_int], ctypes.c_int),
+        "sf_batch_label_counts": ([vp, sz, ctypes.POINTER(sz), ctypes.POINTER(sz),
+                                   ctypes.POINTER(sz)], ctypes.c_int),
+        "sf_batch_fetch_labels": ([vp, sz, vp, sz, vp, sz, vp, sz], ctypes.c_int),
+        "sf_batch_set_timing": ([vp, ctypes.c_int], None),
+        "sf_batch_kernel_ms": ([vp, ctypes.POINTER(ctypes.c_double),
+                                ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
+        "sf_batch_fallbacks": ([vp], ctypes.c_uint64),
+        "sf_labeller_add_batch": ([vp, ctypes.c_int64, vp], ctypes.c_int),
         "sf_grouping_default": ([ctypes.POINTER(Grouping)], None),
         "sf_labeller_create": ([sz, sz, ctypes.POINTER(Grouping),
                                 ctypes.POINTER(vp)], ctypes.c_int),
@@ -230,6 +263,22 @@ class DeviceArray:
             self.free()
         except Exception:
             pass
+
+
+def get_tuning():
+    t = Tuning()
+    lib().sf_get_tuning(ctypes.byref(t))
+    return t.as_dict()
+
+
+def set_tuning(**kw):
+    """Changes the named launch-shape settings (sf_tuning)."""
+    t = Tuning()
+    lib().sf_get_tuning(ctypes.byref(t))
+    for k, v in kw.items():
+        setattr(t, k, int(v))
+    if lib().sf_set_tuning(ctypes.byref(t)) != 0:
+        raise ValueError(last_error())
 
 
 def set_kernel(name):
@@ -365,6 +414,73 @@ def synth_frame(desc, frame=0, out=None):
     return out
 
 
+class Batch:
+    """Many device frames per launch (sf_batch_*): any mix of sizes and of
+    16- and 32-bit frames, optionally replayed from a captured hipGraph."""
+
+    def __init__(self, stream=None):
+        self.h = ctypes.c_void_p()
+        _check(lib().sf_batch_create(stream, ctypes.byref(self.h)))
+        self.n = 0
+
+    def close(self):
+        if self.h:
+            lib().sf_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def threshold(self, frames, graph=False):
+        """frames: (device pointer, height, width, bits) tuples."""
+        arr = (FrameDesc * len(frames))()
+        for i, (p, h, w, b) in enumerate(frames):
+            arr[i].frame, arr[i].height, arr[i].width, arr[i].bits = p, h, w, b
+        _check(lib().sf_batch_threshold(self.h, arr, len(frames), int(bool(graph))))
+        self.n = len(frames)
+
+    def sync(self):
+        """(counts, statuses) per frame."""
+        counts = (ctypes.c_size_t * self.n)()
+        status = (ctypes.c_int * self.n)()
+        _check(lib().sf_batch_sync(self.h, counts, status))
+        return list(counts), list(status)
+
+    def fetch(self, i, n):
+        out = np.zeros(n, SIGNAL_PIXEL)
+        _check(lib().sf_batch_fetch(self.h, i, _ptr(out), n))
+        return out
+
+    def label(self, link_previous=True):
+        _check(lib().sf_batch_label(self.h, int(bool(link_previous))))
+
+    def labels(self, i):
+        n, nc, nl = ctypes.c_size_t(0), ctypes.c_size_t(0), ctypes.c_size_t(0)
+        _check(lib().sf_batch_label_counts(self.h, i, ctypes.byref(n),
+                                           ctypes.byref(nc), ctypes.byref(nl)))
+        comp_of = np.zeros(n.value, np.uint32)
+        comps = np.zeros(nc.value, COMPONENT)
+        links = np.zeros(2 * nl.value, np.uint32)
+        _check(lib().sf_batch_fetch_labels(self.h, i, _ptr(comp_of), n.value,
+                                           _ptr(comps), nc.value, _ptr(links),
+                                           nl.value))
+        return comp_of, comps, links.reshape(-1, 2)
+
+    def set_timing(self, on):
+        lib().sf_batch_set_timing(self.h, int(bool(on)))
+
+    def kernel_ms(self):
+        a, b = ctypes.c_double(0), ctypes.c_double(0)
+        _check(lib().sf_batch_kernel_ms(self.h, ctypes.byref(a), ctypes.byref(b)))
+        return a.value, b.value
+
+    def fallbacks(self):
+        return int(lib().sf_batch_fallbacks(self.h))
+
+
 class LabellerError(RuntimeError):
     """dials_spots::Labeller's std::runtime_error, through the C ABI."""
 
@@ -404,6 +520,9 @@ class Labeller:
 
     def add_workspace(self, frame, ws):
         self._status(lib().sf_labeller_add_ws(self.h, int(frame), ws.h))
+
+    def add_batch(self, first_frame, batch):
+        self._status(lib().sf_labeller_add_batch(self.h, int(first_frame), batch.h))
 
     def add_labelled(self, frame, signal, comp_of, comps, links,
                      links_to_previous=True):

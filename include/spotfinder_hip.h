@@ -113,6 +113,23 @@ enum { SF_KERNEL_FUSED = 0, SF_KERNEL_DIRECT = 1 };
 void sf_set_kernel(int kernel);
 int sf_get_kernel(void);
 
+/* The fused kernel's launch shape, which the autotuner writes.  A wave sweeps
+ * a 256-column strip down `seg_rows` rows (plus 20 rows of halo); seg_rows 0
+ * sizes segments so that one frame makes about `target_waves` waves.
+ * `waves_per_block` is 1, 2 or 4; `frames_per_launch` is the batched mode's
+ * batch.  No counterpart in the reference, whose launch shape is fixed
+ * (dext_cuda.cu:728-730).  From SPOTFINDER_HIP_TUNE, e.g.
+ * "seg_rows=128,target_waves=4096,waves_per_block=2,frames_per_launch=32",
+ * when set.  sf_set_tuning returns -1 for a value out of range. */
+typedef struct sf_tuning {
+  int seg_rows;
+  int target_waves;
+  int waves_per_block;
+  int frames_per_launch;
+} sf_tuning;
+void sf_get_tuning(sf_tuning *t);
+int sf_set_tuning(const sf_tuning *t);
+
 /* Per-stage times of this thread's last profiled find, gpu::StageTimes
  * (src/spots/dext_gpu.hh:89-112), in milliseconds. */
 typedef struct sf_stage_times {
@@ -194,6 +211,47 @@ const sf_signal_pixel *sf_ws_device_signal(sf_workspace *ws);
 const uint32_t *sf_ws_device_mask(sf_workspace *ws, uint32_t *words_per_row);
 
 /* ------------------------------------------------------------------------
+ * Batches.  New surface, no counterpart in the reference, which thresholds
+ * one frame per call: many device frames -- any mix of sizes and of 16- and
+ * 32-bit pixels -- thresholded in one persistent launch of the fused kernel,
+ * each frame's signal pixels ordered, and the frames labelled, in a fixed
+ * number of launches whatever the batch size.  With use_graph the sequence
+ * is captured into a hipGraph the first time a batch of these shapes is seen
+ * and replayed after.  Two batches alternate in the batch's buffers, so the
+ * previous batch stays readable while the next one runs, and frame 0 of a
+ * batch can be linked to the last frame of the one before.
+ * ------------------------------------------------------------------------ */
+
+typedef struct sf_frame_desc {
+  const void *frame; /* device pointer */
+  uint32_t height, width;
+  int32_t bits;      /* 16 or 32 */
+} sf_frame_desc;
+
+typedef struct sf_batch sf_batch;
+int sf_batch_create(void *stream, sf_batch **out);
+void sf_batch_destroy(sf_batch *b);
+void *sf_batch_stream(sf_batch *b);
+/* Enqueues; returns at once.  At most 4096 frames. */
+int sf_batch_threshold(sf_batch *b, const sf_frame_desc *frames, size_t n,
+                       int use_graph);
+/* Waits; counts[i] signal pixels and status[i] 0 or -2 per frame. */
+int sf_batch_sync(sf_batch *b, size_t *counts, int *status);
+int sf_batch_fetch(sf_batch *b, size_t i, sf_signal_pixel *out, size_t n);
+/* Enqueues the labelling of the last batch, each frame linked to the one
+ * before it (frame 0 to the previous batch's last) when link_previous is set
+ * and the shapes agree. */
+int sf_batch_label(sf_batch *b, int link_previous);
+int sf_batch_label_counts(sf_batch *b, size_t i, size_t *n, size_t *ncomp,
+                          size_t *nlinks);
+int sf_batch_fetch_labels(sf_batch *b, size_t i, uint32_t *comp_of, size_t n,
+                          sf_component *comps, size_t ncomp, uint32_t *links,
+                          size_t nlinks);
+void sf_batch_set_timing(sf_batch *b, int on);
+int sf_batch_kernel_ms(sf_batch *b, double *threshold_ms, double *order_ms);
+uint64_t sf_batch_fallbacks(sf_batch *b);
+
+/* ------------------------------------------------------------------------
  * Grouping into spots: dials_spots::Labeller (src/spots/dials_spots.hh:
  * 108-158) with the pixel-level labelling and reductions on the device.
  *
@@ -222,6 +280,8 @@ void sf_labeller_destroy(sf_labeller *l);
 int sf_labeller_add(sf_labeller *l, int64_t frame,
                     const sf_signal_pixel *signal, size_t n);
 int sf_labeller_add_ws(sf_labeller *l, int64_t frame, sf_workspace *ws);
+/* Every frame of a labelled batch, numbered first_frame, first_frame+1, ... */
+int sf_labeller_add_batch(sf_labeller *l, int64_t first_frame, sf_batch *b);
 /* A frame labelled elsewhere (a batched launch, another device): its
  * ordered list, each pixel's component (numbered in order of first pixel),
  * the components, and (here, previous-frame) component pairs, honoured when

@@ -1,0 +1,142 @@
+"""Batches: many frames per persistent launch, on the device.
+
+  * an interleaved batch of frame sizes and pixel types -- EIGER2 4M, 9M and
+    16M uint32 frames, 512 x 512 uint16 ROIs, and the golden edge matrix --
+    thresholded in one launch, every frame bit-exact against the reference's
+    own output (golden vectors) or the oracle;
+  * the same through a captured hipGraph, replayed several times with other
+    frames in between eager work, every replay checked;
+  * a frame over the list's capacity (-2) and a frame beyond the fused
+    kernel's exact range (re-run exactly) inside a batch, the others intact;
+  * batch labelling, linked across batch boundaries, grouped into spots, equal
+    to the oracle's Labeller on the oracle's threshold.
+"""
+import hashlib
+import sys
+
+import numpy as np
+import pytest
+
+from labeller_cases import compare_spots
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def frame_of(device, case):
+    kind = case["kind"] if isinstance(case["kind"], str) else tuple(case["kind"])
+    f = device.synth_frame(device.synth_desc(kind, **case["overrides"]), case["frame"])
+    assert hashlib.sha256(f.tobytes()).hexdigest() == case["frame_sha256"]
+    return f
+
+
+def mixed_frames(device, synth_cases, matrix):
+    """(name, host frame, expected check) in an interleaved order."""
+    out = []
+    names = ["eiger4m_f0", "roi512_u16", "eiger9m_f0", "roi512_u16",
+             "eiger16m_f0", "eiger4m_f7_seed3", "roi512_u16", "eiger4m_dark"]
+    for nm in names:
+        out.append((nm, frame_of(device, synth_cases[nm]), synth_cases[nm]))
+    for nm in ("m257x511_2.0_32", "m67x300_12.0_16", "m1x1_0.3_16",
+               "m33x65_2.0_32"):
+        f, expected = matrix[nm]
+        out.append((nm, f, expected))
+    return out
+
+
+def check_frame(name, got, expected):
+    if isinstance(expected, dict):
+        assert len(got) == expected["count"], name
+        assert hashlib.sha256(got.tobytes()).hexdigest() == expected["signal_sha256"], name
+    else:
+        assert got.tobytes() == expected.tobytes(), name
+
+
+def run_batch(device, batch, frames, graph):
+    dev = [device.DeviceArray.from_numpy(f) for _, f, _ in frames]
+    descs = [(d.ptr, f.shape[0], f.shape[1], 16 if f.dtype == np.uint16 else 32)
+             for d, (_, f, _) in zip(dev, frames)]
+    batch.threshold(descs, graph=graph)
+    counts, status = batch.sync()
+    for i, (name, f, expected) in enumerate(frames):
+        assert status[i] == 0, name
+        check_frame(name, batch.fetch(i, counts[i]), expected)
+    return dev
+
+
+@pytest.mark.parametrize("graph", [False, True])
+def test_mixed_batch_bit_exact(device, synth_cases, matrix, graph):
+    frames = mixed_frames(device, synth_cases, matrix)
+    batch = device.Batch()
+    say("mixed batch", len(frames), "graph", graph)
+    run_batch(device, batch, frames, graph)
+    if graph:
+        # Replays with other frames of the same shapes, eager work between.
+        for rep in range(3):
+            shuffled = frames[:]
+            # swap the two 4M frames and the roi copies: same shapes,
+            # different contents per slot
+            i0, i1 = 0, 5
+            shuffled[i0], shuffled[i1] = shuffled[i1], shuffled[i0]
+            if rep % 2 == 0:
+                frames = shuffled
+            say("replay", rep)
+            run_batch(device, batch, frames, True)
+            ws = device.Workspace()
+            f = frames[0][1]
+            d = device.DeviceArray.from_numpy(f)
+            ws.threshold(d.ptr, 32, f.shape[0], f.shape[1], "fused")
+            assert ws.sync()[0] == 0
+            ws.close()
+    batch.close()
+
+
+def test_batch_with_overfull_and_wide_frames(device, synth_cases):
+    dense = np.zeros((256, 512), np.uint32)
+    dense[:, ::2] = 10000
+    wide = frame_of(device, synth_cases["wide_values_u32"])
+    ok = frame_of(device, synth_cases["eiger4m_f0"])
+    frames = [dense, wide, ok]
+    dev = [device.DeviceArray.from_numpy(f) for f in frames]
+    batch = device.Batch()
+    batch.threshold([(d.ptr, f.shape[0], f.shape[1], 32)
+                     for d, f in zip(dev, frames)])
+    counts, status = batch.sync()
+    assert status == [-2, 0, 0]
+    check_frame("wide", batch.fetch(1, counts[1]), synth_cases["wide_values_u32"])
+    check_frame("ok", batch.fetch(2, counts[2]), synth_cases["eiger4m_f0"])
+    assert batch.fallbacks() >= 1
+    batch.close()
+
+
+@pytest.mark.parametrize("mode", ["3d", "2d"])
+def test_batch_label_group(device, mode):
+    desc = device.synth_desc("4m", spots_per_frame=600, persist=1, seed=91)
+    h, w = desc.height, desc.width
+    host = [device.synth_frame(desc, z) for z in range(7)]
+    dev = [device.DeviceArray.from_numpy(f) for f in host]
+    lab = device.Labeller(h, w, two_d=(mode == "2d"))
+    ref = O.oracle().labeller(h, w, two_d=(mode == "2d"))
+    batch = device.Batch()
+    for start, stop in ((0, 3), (3, 6), (6, 7)):
+        say(mode, "batch", start, stop)
+        batch.threshold([(dev[z].ptr, h, w, 32) for z in range(start, stop)],
+                        graph=(start == 3))
+        counts, status = batch.sync()
+        assert status == [0] * (stop - start)
+        batch.label(True)
+        lab.add_batch(start, batch)
+    for z in range(7):
+        _, want = O.oracle().dext(host[z])
+        ref.add(z, want)
+    lab.finish()
+    ref.finish()
+    compare_spots(lab.spots(), ref.spots())
+    assert lab.pixels().tobytes() == ref.pixels().tobytes()
+    assert lab.counts() == ref.counts()
+    batch.close()
