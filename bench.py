@@ -1,16 +1,21 @@
 """Benchmark of the spot-finding hot path on one MI355X.
 
-Workload (EIGER2-XE 16M sweep): 4148 x 4362 uint32 synthetic frames, Poisson
-background 0.8 with a solvent ring, 2000 spots per frame persisting 1-3
-frames, 8 x 4 module gaps masked, 0.01 % hot pixels; the dispersion threshold
-(7x7 / erode 5x5 / 11x11), ordered SignalPixel list and strong-pixel bitmask,
-one frame per launch sequence ("plain per-frame" mode).  Frames are generated
-on the host, uploaded once, and are resident in HBM when the timed region
-starts.  A step is one pass over the batch of frames.
+Workload: EIGER2-XE 16M frames, 4148 x 4362 uint32, synthetic -- Poisson
+background 0.8 with a solvent ring (+1.5 at r = 600 px, sigma 40), 2000
+Gaussian spots per frame (sigma 0.6-1.5 px, peak 20-5000) persisting 1-3
+frames, 8 x 4 module gaps masked, 0.01 % hot pixels at 65535.  The path:
+the dispersion threshold (7x7 / erode 5x5 / 11x11) into an ordered
+SignalPixel list and a strong-pixel bitmask, then four-connected labelling
+with exact per-component moments (2D), all on the device.
 
-Prints one JSON line:  frames/s (value), the fused kernel's achieved HBM
-bandwidth against the 8 TB/s peak (roofline), and the reference's CPU path on
-this host's cores over a bounded sample of the same frames (cpu_baseline).
+Frames are generated on the host, uploaded once and resident in HBM before
+the timed region.  A step is one batch of frames (--frames) through one
+persistent threshold launch, the batched ordering and the batched labelling;
+`value` is frames per second over the timed steps.  Beside it: the plain
+per-frame mode (one frame per launch sequence, waiting for each), the fused
+kernel's achieved HBM bandwidth against the 8 TB/s peak from HIP events on
+its stream, and the reference's own CPU threshold (compiled from its
+sources, oracle/_ref) on this host's cores over a bounded sample.
 """
 import argparse
 import json
@@ -25,44 +30,48 @@ sys.path.insert(0, ROOT)
 sys.path.insert(0, os.path.join(ROOT, "dials-metal_amd"))
 
 HBM_PEAK_GBS = 8000.0
-METRIC = "16M frames/s + achieved HBM GB/s (bit-exact mask); serving p50/p99 frame latency"
+METRIC = ("16M frames/s + achieved HBM GB/s (bit-exact mask); "
+          "serving p50/p99 frame latency")
 
 
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(frames_host, h, w):
+def cpu_baseline(frames_host, threads):
     """The reference's dext<uint32_t> (oracle/_ref, compiled from the
     reference's own sources) on this host's cores, a bounded sample."""
     from oracle import oracle as O
-    threads = min(16, os.cpu_count() or 1)
     lib, kind = O.reference(), "reference"
     if lib is None:
         lib, kind = O.oracle(), "port"
-    n = min(len(frames_host), max(threads, 8))
+    n = min(len(frames_host), 2 * threads)
     sample = np.stack(frames_host[:n])
     signal, secs = lib.threads_u32(sample, threads)
     return {"value": n / secs, "unit": "frames/s", "cores": threads, "kind": kind,
-            "sample": f"{n} of the benchmark's 16M frames, dext<uint32_t> on "
-                      f"{threads} threads, one frame per thread at a time "
-                      f"({secs:.2f} s wall, {signal} signal pixels)"}
+            "sample": f"{n} of the benchmark's 16M uint32 frames through "
+                      f"dext<uint32_t> on {threads} threads, one frame per "
+                      f"thread at a time: {secs:.2f} s wall, {signal} signal "
+                      f"pixels; threshold only, no grouping"}
 
 
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--frames", type=int, default=16,
-                    help="distinct frames resident in HBM, one step's batch")
-    ap.add_argument("--kernel", default="fused", choices=["fused", "direct"])
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=3)
+    ap.add_argument("--frames", type=int, default=32,
+                    help="frames per batch (one step), resident in HBM")
+    ap.add_argument("--graph", type=int, default=1,
+                    help="replay the batch's threshold from a captured hipGraph")
+    ap.add_argument("--no-label", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build has no multi-GPU path "
-              f"(asked for --gpus {args.gpus}); see DESIGN.md for how frames "
-              f"would shard", file=sys.stderr)
+              f"(asked for --gpus {args.gpus}); DESIGN.md describes how the "
+              f"frames would shard", file=sys.stderr)
         sys.exit(2)
 
     import spotfinder_hip as sf
@@ -74,44 +83,46 @@ def main():
     log(f"generating {args.frames} frames of {w} x {h} uint32")
     frames_host = [sf.synth_frame(desc, z) for z in range(args.frames)]
     dev = [sf.DeviceArray.from_numpy(f) for f in frames_host]
+    descs = [(d.ptr, h, w, 32) for d in dev]
     sf.synchronize()
+    label = not args.no_label
 
-    ws = sf.Workspace()
-    ws.set_timing(True)
+    batch = sf.Batch()
     counts = []
 
-    def step(record):
-        kms = []
-        for i, d in enumerate(dev):
-            ws.threshold(d.ptr, 32, h, w, args.kernel)
-            rc, n = ws.sync()
-            if rc != 0:
-                raise RuntimeError(f"frame {i}: status {rc}")
-            if record:
-                kms.append(ws.kernel_ms()[0])
-                counts.append(n)
-        return kms
+    def step(graph=True):
+        batch.threshold(descs, graph=graph)
+        if label:
+            batch.label(True)
+        c, status = batch.sync()
+        if any(status):
+            raise RuntimeError(f"a frame was refused: {status}")
+        return c
 
     for _ in range(args.warmup):
-        step(False)
-    ws.set_timing(False)
+        counts = step(bool(args.graph))
     sf.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
-        step(False)
+        step(bool(args.graph))
     sf.synchronize()
     elapsed = time.perf_counter() - t0
+    frames_done = args.steps * len(dev)
 
-    # Kernel durations, measured live with HIP events on the kernel's stream.
-    ws.set_timing(True)
-    kms = step(True)
-    kernel_ms = float(np.mean(kms))
+    # The fused kernel's duration, from HIP events on the batch's stream.
+    batch.set_timing(True)
+    kms = []
+    for _ in range(3):
+        batch.threshold(descs)
+        batch.sync()
+        kms.append(batch.kernel_ms()[0])
+    batch.set_timing(False)
+    kernel_ms = float(np.median(kms))
     mean_signal = float(np.mean(counts))
     n_px = h * w
-    algo_bytes = n_px * 4 + 16 * mean_signal + n_px / 8.0
-    achieved = algo_bytes / (kernel_ms * 1e-3) / 1e9
+    bytes_per_frame = n_px * 4 + 16 * mean_signal + n_px / 8.0
+    achieved = len(dev) * bytes_per_frame / (kernel_ms * 1e-3) / 1e9
 
-    frames_done = args.steps * len(dev)
     line = {
         "metric": METRIC,
         "value": frames_done / elapsed,
@@ -124,19 +135,45 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic",
-        "config": {"workload": "EIGER2-XE 16M sweep frames, plain per-frame "
-                               "threshold (one launch sequence per frame)",
-                   "frame": [h, w], "pixel": "uint32",
-                   "frames_per_step": len(dev), "kernel": args.kernel,
-                   "mean_signal_pixels": mean_signal},
-        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": None,
-                     "kernel": f"fused_kernel<uint32_t>, {kernel_ms:.4f} ms/frame"},
+        "data": "synthetic (EIGER2-XE 16M generator, seed 2024, resident in HBM)",
+        "config": {
+            "workload": "EIGER2-XE 16M uint32 frames (4148 x 4362), batched: "
+                        "one persistent fused threshold launch per batch, "
+                        "ordered SignalPixel lists, strong-pixel bitmasks"
+                        + (", 2D labelling with exact per-component moments"
+                           if label else ""),
+            "frame": [h, w], "pixel": "uint32", "frames_per_step": len(dev),
+            "graph": bool(args.graph), "tuning": sf.get_tuning(),
+            "mean_signal_pixels": mean_signal},
+        "roofline": {
+            "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "kernel": "fused_kernel (one launch per batch)",
+            "ms_per_launch": kernel_ms,
+            "bytes_per_frame": bytes_per_frame},
     }
+
+    if not args.no_plain:
+        # One frame per launch sequence, waiting for each: gpu::find's shape
+        # with the frame already in HBM.
+        ws = sf.Workspace()
+        for d in dev[:2]:
+            ws.threshold(d.ptr, 32, h, w, "fused")
+            ws.sync()
+        t0 = time.perf_counter()
+        for d in dev:
+            ws.threshold(d.ptr, 32, h, w, "fused")
+            rc, _ = ws.sync()
+            if label:
+                sf.workspace_label_enqueue(ws)
+                ws.sync()
+        plain = len(dev) / (time.perf_counter() - t0)
+        ws.close()
+        line["plain_per_frame"] = {"value": plain, "unit": "frames/s",
+                                   "labelled": label}
+
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(frames_host, h, w)
+        line["cpu_baseline"] = cpu_baseline(frames_host, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)
 
 

@@ -225,12 +225,17 @@ Engine::Prepared Engine::prepare(const FrameIn *frames, const int *ids,
   FusedParams *fp = reinterpret_cast<FusedParams *>(host);
   FrameDev *fd = reinterpret_cast<FrameDev *>(host + p.b_fused);
   ScanSeg *sg = reinterpret_cast<ScanSeg *>(host + p.b_fused + p.b_frames);
+  uint64_t strip_rows = 0;
+  for (size_t i = 0; i < n; i++)
+    strip_rows += static_cast<uint64_t>(slot(ids[i]).height) *
+                  strips_of(slot(ids[i]).width);
+  const uint32_t seg = segment_rows(strip_rows, t);
   for (size_t i = 0; i < n; i++) {
     FrameSlot &s = slot(ids[i]);
     s.frame = frames[i].frame;
     s.kernel = kernel;
     FusedPlan plan;
-    const uint32_t k = plan_fused(plan, s.height, s.width, t);
+    const uint32_t k = plan_fused(plan, s.height, s.width, seg);
     fp[i] = to_params(args(ids[i], frames[i].frame), plan, p.items);
     p.items += k;
     fd[i] = dev(ids[i], -1);

@@ -147,7 +147,8 @@ __device__ __forceinline__ uint32_t component_of(const FrameDev &f, uint32_t r) 
          static_cast<uint32_t>(__popc(f.roots[r / 32] & ((1u << (r & 31u)) - 1u)));
 }
 
-// Component of every pixel; each root opens its component's record.
+// Component of every pixel; each root opens its component's record, and the
+// last pixel, which knows how many roots precede it, writes the count.
 __global__ __launch_bounds__(256) void number_components(
     const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
@@ -161,8 +162,9 @@ __global__ __launch_bounds__(256) void number_components(
       z.first = f.ordered[i].index;
       z.x0 = z.y0 = 0xFFFFFFFFu;
       f.comps[c] = z;
-      atomicAdd(&f.counter[2], 1u);
     }
+    if (i == n - 1)
+      f.counter[2] = component_of(f, i) + (r == i ? 1u : 0u);
   }
 }
 
@@ -176,31 +178,66 @@ __device__ __forceinline__ void add128(uint64_t *pair, uint64_t lo, uint64_t hi)
               static_cast<unsigned long long>(hi + carry));
 }
 
-// Each pixel's exact moments into its component.
+// Adds a 128-bit value held as (lo, hi) to a running (lo, hi) in registers.
+__device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t l,
+                                       uint64_t h) {
+  const uint64_t s = lo + l;
+  hi += h + (s < lo ? 1u : 0u);
+  lo = s;
+}
+
+// Exact moments into the components, one row-run at a time: the pixel that
+// starts a run (no set pixel to its left) sums the run in registers, then
+// adds the run to its component with one set of atomics.  A run lies in one
+// row and one component, and runs are short, so the component records see a
+// few atomics per row of a spot rather than one set per pixel.
 __global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   const uint32_t n = live_count(f);
   FOR_EACH_PIXEL(i, n) {
-    const sf_signal_pixel p = f.ordered[i];
-    const uint32_t y = p.index / f.width, x = p.index - y * f.width;
+    const uint32_t first = f.ordered[i].index;
+    const uint32_t y = first / f.width, x0 = first - y * f.width;
+    if (x0 > 0 && i > 0 && f.ordered[i - 1].index == first - 1)
+      continue; // not the start of a run
+    const uint64_t ay = 2ull * y + 1ull;
+    uint32_t count = 0, x = x0;
+    uint64_t peak = 0, sum = 0;
+    uint64_t sq_lo = 0, sq_hi = 0, sx_lo = 0, sx_hi = 0, xx_lo = 0, xx_hi = 0;
+    uint32_t j = i;
+    for (;;) {
+      const uint64_t v = f.ordered[j].value;
+      const uint64_t ax = 2ull * x + 1ull;
+      const uint64_t key = (v << 32) | (0xFFFFFFFFu - (first + count));
+      peak = key > peak ? key : peak;
+      sum += v;
+      acc128(sq_lo, sq_hi, v * v, 0);
+      acc128(sx_lo, sx_hi, v * ax, 0);
+      const uint64_t xx = ax * ax;
+      acc128(xx_lo, xx_hi, v * xx, __umul64hi(v, xx));
+      count++;
+      j++;
+      x++;
+      if (j >= n || x >= f.width || f.ordered[j].index != first + count)
+        break;
+    }
     sf_component *c = &f.comps[f.comp_of[i]];
-    const uint64_t v = p.value;
-    const uint64_t ax = 2ull * x + 1ull, ay = 2ull * y + 1ull;
-    atomicAdd(&c->n, 1u);
-    atomicMin(&c->x0, x);
-    atomicMax(&c->x1, x + 1);
+    atomicAdd(&c->n, count);
+    atomicMin(&c->x0, x0);
+    atomicMax(&c->x1, x0 + count);
     atomicMin(&c->y0, y);
     atomicMax(&c->y1, y + 1);
     atomicMax(reinterpret_cast<unsigned long long *>(&c->peak),
-              static_cast<unsigned long long>((v << 32) | (0xFFFFFFFFu - p.index)));
+              static_cast<unsigned long long>(peak));
     atomicAdd(reinterpret_cast<unsigned long long *>(&c->sum),
-              static_cast<unsigned long long>(v));
-    add128(c->sum_sq, v * v, 0);
-    add128(c->sum_x, v * ax, 0);
-    add128(c->sum_y, v * ay, 0);
-    const uint64_t xx = ax * ax, yy = ay * ay;
-    add128(c->sum_xx, v * xx, __umul64hi(v, xx));
-    add128(c->sum_yy, v * yy, __umul64hi(v, yy));
+              static_cast<unsigned long long>(sum));
+    add128(c->sum_sq, sq_lo, sq_hi);
+    add128(c->sum_x, sx_lo, sx_hi);
+    // sum of value * ay and value * ay^2 over the run: ay is the run's
+    // constant, so these are products of the run's sum.
+    add128(c->sum_y, sum * ay, __umul64hi(sum, ay));
+    const uint64_t yy = ay * ay;
+    add128(c->sum_yy, sum * yy, __umul64hi(sum, yy));
+    add128(c->sum_xx, xx_lo, xx_hi);
   }
 }
 
@@ -214,20 +251,33 @@ __global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ 
   FOR_EACH_PIXEL(i, n) {
     const uint32_t idx = f.ordered[i].index;
     const uint32_t y = idx / f.width, x = idx - y * f.width;
-    if (!bit_set(f.prev_mask, f.wpr, x, y))
+    bool emit = false;
+    uint32_t here = 0, there = 0;
+    if (bit_set(f.prev_mask, f.wpr, x, y)) {
+      here = f.comp_of[i];
+      there = f.prev_comp_of[rank_of(f.prev_mask, f.prev_prefix, f.wpr, x, y)];
+      emit = !(x > 0 && i > 0 && f.ordered[i - 1].index == idx - 1 &&
+               f.comp_of[i - 1] == here && bit_set(f.prev_mask, f.wpr, x - 1, y) &&
+               f.prev_comp_of[rank_of(f.prev_mask, f.prev_prefix, f.wpr, x - 1,
+                                      y)] == there);
+    }
+    // One atomic on the frame's link count per wave, not per link.
+    const uint64_t b = __ballot(emit);
+    if (b == 0)
       continue;
-    const uint32_t here = f.comp_of[i];
-    const uint32_t there =
-        f.prev_comp_of[rank_of(f.prev_mask, f.prev_prefix, f.wpr, x, y)];
-    if (x > 0 && i > 0 && f.ordered[i - 1].index == idx - 1 &&
-        f.comp_of[i - 1] == here && bit_set(f.prev_mask, f.wpr, x - 1, y) &&
-        f.prev_comp_of[rank_of(f.prev_mask, f.prev_prefix, f.wpr, x - 1, y)] ==
-            there)
-      continue;
-    const uint32_t at = atomicAdd(&f.counter[3], 1u);
-    if (at < f.link_capacity) {
-      f.links[2 * at] = here;
-      f.links[2 * at + 1] = there;
+    const uint32_t lane = sfd::lane_id();
+    const int leader = __ffsll(static_cast<unsigned long long>(b)) - 1;
+    uint32_t base = 0;
+    if (static_cast<int>(lane) == leader)
+      base = atomicAdd(&f.counter[3], static_cast<uint32_t>(__popcll(b)));
+    base = __shfl(base, leader);
+    if (emit) {
+      const uint32_t at =
+          base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)));
+      if (at < f.link_capacity) {
+        f.links[2 * at] = here;
+        f.links[2 * at + 1] = there;
+      }
     }
   }
 }

@@ -52,7 +52,10 @@ void resolve_kernel() {
 
 std::mutex g_tuning_mutex;
 bool g_tuning_read = false;
-sf_tuning g_tuning = {0, 4096, 2, 32};
+// Measured on MI355X over 16-frame batches of 16M frames: four waves a block
+// (the most resident waves under the LDS each wave's ring takes), segments
+// of 128-256 rows; a lone frame gets 32-row segments to fill the device.
+sf_tuning g_tuning = {0, 8192, 4, 32};
 
 bool tuning_valid(const sf_tuning &t) {
   return t.seg_rows >= 0 && t.seg_rows <= (1 << 20) && t.target_waves >= 1 &&

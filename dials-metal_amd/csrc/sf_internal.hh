@@ -105,8 +105,12 @@ struct FusedParams {
 struct FusedPlan {
   uint32_t strips = 0, segments = 0, seg_rows = 0;
 };
+// Strips of a frame's width; rows per segment for a batch of `strip_rows`
+// (the sum over its frames of height x strips) under a tuning; the plan.
+uint32_t strips_of(uint32_t width);
+uint32_t segment_rows(uint64_t strip_rows, const Tuning &t);
 uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
-                    const Tuning &t);
+                    uint32_t seg_rows);
 FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,
                       uint32_t item_base);
 // One persistent launch over every item of `nframes` frames (descriptors in

@@ -455,21 +455,25 @@ void launch(const FusedParams *frames, uint32_t nframes, uint32_t total_items,
 
 namespace sfi {
 
+uint32_t strips_of(uint32_t width) { return (width + kOut - 1) / kOut; }
+
+uint32_t segment_rows(uint64_t strip_rows, const Tuning &t) {
+  if (t.seg_rows > 0)
+    return static_cast<uint32_t>(t.seg_rows);
+  // Segments tall enough that the 20 halo rows cost little, short enough
+  // that the batch still spreads over the whole device: about target_waves
+  // items, each 32 to 256 rows.
+  const uint64_t target = static_cast<uint64_t>(t.target_waves);
+  uint64_t seg = (strip_rows + target - 1) / target;
+  seg = seg < 32 ? 32 : (seg > 256 ? 256 : seg);
+  return static_cast<uint32_t>(seg);
+}
+
 uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
-                    const Tuning &t) {
-  p.strips = (width + kOut - 1) / kOut;
-  uint32_t seg = static_cast<uint32_t>(t.seg_rows);
-  if (seg == 0) {
-    // Segments tall enough that the 20 halo rows cost little, short enough
-    // that a single frame still spreads over the whole device.
-    const uint32_t target = static_cast<uint32_t>(t.target_waves);
-    seg = static_cast<uint32_t>(
-        (static_cast<uint64_t>(height) * p.strips + target - 1) / target);
-    if (seg < 16)
-      seg = 16;
-  }
-  p.seg_rows = seg;
-  p.segments = (height + seg - 1) / seg;
+                    uint32_t seg_rows) {
+  p.strips = strips_of(width);
+  p.seg_rows = seg_rows;
+  p.segments = (height + seg_rows - 1) / seg_rows;
   return p.strips * p.segments;
 }
 

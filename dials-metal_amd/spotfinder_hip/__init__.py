@@ -562,6 +562,11 @@ class Labeller:
         return int(lib().sf_labeller_live(self.h))
 
 
+def workspace_label_enqueue(ws, link_previous=True):
+    """Enqueues the labelling of the workspace's last frame; returns at once."""
+    _check(lib().sf_ws_label(ws.h, int(bool(link_previous))))
+
+
 def workspace_label(ws, link_previous=True):
     """Labels the workspace's last frame; returns (comp_of, comps, links,
     linked) copied to the host."""
