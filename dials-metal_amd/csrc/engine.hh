@@ -107,6 +107,7 @@ private:
   bool own_stream_ = false;
   std::vector<std::unique_ptr<FrameSlot>> slots_;
   DeviceBuffer counters_, table_, desc_;
+  DeviceBuffer queue_; // the fused kernel's item queue, one word
   uint32_t *host_counters_ = nullptr;
   size_t host_counter_words_ = 0;
   char *staging_[2] = {nullptr, nullptr};

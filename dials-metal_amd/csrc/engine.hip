@@ -47,6 +47,7 @@ Engine::Engine(void *borrowed_stream) {
                   hipMemcpyHostToDevice),
         "uploading the root table");
   desc_.ensure(2 * kDescHalf, stream_);
+  queue_.ensure(256, stream_);
 }
 
 Engine::~Engine() {
@@ -258,14 +259,16 @@ void Engine::enqueue_kernels(const Prepared &p, const FrameIn *frames,
   const ScanSeg *sg_d =
       reinterpret_cast<const ScanSeg *>(d + p.b_fused + p.b_frames);
   const uint32_t nf = static_cast<uint32_t>(n);
-  launch_clear(fd_d, nf, kernel == SF_KERNEL_FUSED, 0, stream_);
+  const bool fused = kernel == SF_KERNEL_FUSED;
+  launch_clear(fd_d, nf, fused, 0, stream_,
+               fused ? queue_.as<uint32_t>() : nullptr);
   if (timing)
     check(hipEventRecord(marks_[0], stream_), "hipEventRecord");
-  if (kernel == SF_KERNEL_FUSED) {
+  if (fused) {
     launch_fused_batch(fp_d, nf, p.items, table_.as<float>(),
                        reinterpret_cast<const double *>(table_.as<char>() +
                                                         128 * sizeof(float)),
-                       stream_);
+                       queue_.as<uint32_t>(), stream_);
   } else {
     for (size_t i = 0; i < n; i++)
       launch_direct(args(ids[i], frames[i].frame), scr0_.as<uint8_t>(),

@@ -156,7 +156,10 @@ __global__ __launch_bounds__(256) void set_bits(const FrameDev *__restrict__ fra
 }
 
 __global__ __launch_bounds__(256) void clear_frames(
-    const FrameDev *__restrict__ frames, int masks, int first) {
+    const FrameDev *__restrict__ frames, int masks, int first,
+    uint32_t *__restrict__ queue) {
+  if (queue && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
+    *queue = 0;
   const FrameDev f = frames[blockIdx.y];
   if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) >= first &&
       threadIdx.x < 4)
@@ -197,11 +200,11 @@ void launch_popcount_scan(const ScanSeg *segs_device, uint32_t nframes,
 }
 
 void launch_clear(const FrameDev *frames_device, uint32_t nframes, bool masks,
-                  int first, hipStream_t stream) {
+                  int first, hipStream_t stream, uint32_t *queue) {
   if (nframes == 0)
     return;
   clear_frames<<<dim3(masks ? 256 : 1, nframes), 256, 0, stream>>>(
-      frames_device, masks ? 1 : 0, first);
+      frames_device, masks ? 1 : 0, first, queue);
   check(hipGetLastError(), "launching the frame clear");
 }
 

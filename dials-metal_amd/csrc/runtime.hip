@@ -325,6 +325,13 @@ int sf_copy_to_host(void *dst, const void *src, size_t bytes) {
   });
 }
 
+int sf_copy_on_device(void *dst, const void *src, size_t bytes) {
+  return guarded([&]() -> int {
+    check(hipMemcpy(dst, src, bytes, hipMemcpyDeviceToDevice), "hipMemcpy");
+    return 0;
+  });
+}
+
 int sf_device_synchronize(void) {
   return guarded([&]() -> int {
     check(hipDeviceSynchronize(), "hipDeviceSynchronize");

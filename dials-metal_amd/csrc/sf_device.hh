@@ -31,6 +31,15 @@ template <> struct traits<uint32_t> {
   static constexpr uint32_t masked = 0xFFFFFFFEu;
 };
 
+// Global-memory views of pointers that arrive through descriptors in memory,
+// so that accesses through them compile to global rather than flat
+// instructions: a flat access also counts against the LDS wait counter, which
+// makes every wait for an LDS read wait for loads in flight too.
+#define SF_GLOBAL __attribute__((address_space(1)))
+template <typename P> __device__ __forceinline__ SF_GLOBAL P *gmem(P *p) {
+  return (SF_GLOBAL P *)p;
+}
+
 __device__ __forceinline__ float root(float v) { return __fsqrt_rn(v); }
 __device__ __forceinline__ double root(double v) { return __dsqrt_rn(v); }
 __device__ __forceinline__ float quotient(float a, float b) {

@@ -115,10 +115,11 @@ FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,
                       uint32_t item_base);
 // One persistent launch over every item of `nframes` frames (descriptors in
 // device memory), mixed sizes and pixel types; each frame's mask and
-// overflow word cleared beforehand.
+// overflow word cleared beforehand, and `queue`, the device word from which
+// waves take items, zeroed.
 void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
                         uint32_t total_items, const float *tf, const double *td,
-                        hipStream_t stream);
+                        uint32_t *queue, hipStream_t stream);
 
 // Clears `n` 32-bit words with a kernel, which unlike a memset node stays
 // correct when the sequence is captured into a graph and replayed.
@@ -160,9 +161,9 @@ struct FrameDev {
 };
 
 // Clears each frame's counters from `first` to the fourth and, with `masks`,
-// its bitmask.
+// its bitmask; zeroes `queue` when given (the fused kernel's item queue).
 void launch_clear(const FrameDev *frames_device, uint32_t nframes, bool masks,
-                  int first, hipStream_t stream);
+                  int first, hipStream_t stream, uint32_t *queue = nullptr);
 // Places every frame's unordered emit at its rank (blockIdx.y = frame).
 void launch_scatter(const FrameDev *frames_device, uint32_t nframes,
                     hipStream_t stream);

@@ -8,31 +8,52 @@
 // erode (halo 2 more), which needs stage 0's 7 x 7 sums (halo 3 more), so the
 // strip carries 12 columns of halo on each side (10 needed, rounded to a lane)
 // and writes the middle 232; rows run from 10 above the segment to 10 below.
-// Pixels outside the frame load as masked, which is exact (a masked pixel adds
-// nothing to a sum, is set in scr0 so it cannot break an erode, and stays out
-// of stage 2's background), the same argument as dext_fused.hh:16-24.
+// Pixels outside the frame count as masked, which is exact (a masked pixel
+// adds nothing to a sum, is set in scr0 so it cannot break an erode, and stays
+// out of stage 2's background), the same argument as dext_fused.hh:16-24.
+//
+// Representation.  A masked pixel is held as the value 0 with its bit in a
+// register history of masked nibbles (one nibble a row, a bit a column), so
+// every window sum is a plain sum of values and a window's count is the
+// window's size less the masked bits in it.  The entering row is read from
+// HBM two rows ahead of use; the rows leaving the windows are read back from a
+// 16-row ring in LDS that holds each pixel's low 16 bits, with bits 16-17 in a
+// byte ring written and read only for the rare rows that have such a pixel (a
+// wave-uniform row mask says which).  Sixteen rows suffice because the row 16
+// above is read before the entering row takes its slot.  The ring starts at
+// zero, so the rows before the first read back as nothing and no step needs a
+// prologue test.  With a 48-entry candidate stage a wave holds 9984 bytes of
+// LDS, so that four blocks of four waves fit a CU's 160 KB.
 //
 // Per row entering at the bottom, the wave
-//   * updates stage 0's vertical 7-row sums per column (registers), with the
-//     row leaving at the top read back from a 17-row ring of raw pixels in LDS;
-//   * sums 7 columns across lanes (DPP lane shifts), tests, and so finishes
-//     scr0 three rows up;
-//   * erodes with bit operations on nibbles of scr0 (5 columns by 5 rows), so
+//   * updates stage 0's vertical 7-row sums of values and of squares per
+//     column (registers), with the square's change as one exact product
+//     (in - out)(in + out);
+//   * sums 7 columns across lanes (DPP lane shifts) and tests, finishing scr0
+//     three rows up.  Where no masked pixel lies in any of the wave's windows
+//     (a wave-uniform test) every count is 49; elsewhere the counts come from
+//     the masked history by popcount and travel in the top bits of the sums;
+//   * erodes with bit operations on nibbles of scr0 (5 columns by 5 rows),
 //     finishing scr1 five rows up;
-//   * updates stage 2's vertical 11-row sums of pixels that are valid and not
-//     scr1, sums 11 columns across lanes, and tests the candidates ten rows up,
-//     emitting signal pixels with one atomic per wave and setting their bits
-//     in the strong-pixel bitmask.
+//   * updates stage 2's vertical 11-row sums of values outside scr1 (only
+//     unmasked scr1 pixels need excluding, the masked ones being zero, and a
+//     wave-uniform test skips even that for rows with none), and, on rows with
+//     a candidate ten rows up, forms the counts from the histories, sums 11
+//     columns across lanes and tests, emitting signal pixels to a per-wave LDS
+//     stage (one atomic per flush) and setting their bits in the strong-pixel
+//     bitmask.
 //
-// Arithmetic.  The count and the sum of a window travel together in one
-// 32-bit word (count in the top bits), the sum of squares in a double; both
-// are exact while every valid pixel is below 2^18 (49 * (2^18-1) < 2^26 under
-// a count at bit 26; 121 * (2^18-1) < 2^25 under a count at bit 25; sums of
-// squares below 2^53).  A wave that meets a larger valid pixel raises the
+// Arithmetic.  Sums of values are exact in 32 bits while every valid pixel is
+// below 2^18 (49 * (2^18-1) < 2^26 under a count at bit 26; 121 * (2^18-1) <
+// 2^25 under a count at bit 25); squares below 2^36 and their sums below 2^53
+// are exact doubles.  A wave that meets a larger valid pixel raises the
 // frame's overflow flag and the caller re-runs that frame with the direct
 // kernels, whose sums are in the reference's own wide types.  uint16 frames
 // never overflow.  The tests are the reference's expressions in its real type
-// on the exact sums; the stage-0 root comes from a host-computed table.
+// on the exact sums; for uint32 frames every intermediate of the left side is
+// an integer below 2^53, so it is formed with a fused multiply-add that is
+// exact, and the right side keeps the reference's two roundings; the stage-0
+// root comes from a host-computed table.
 #include "sf_device.hh"
 #include "sf_internal.hh"
 
@@ -45,9 +66,11 @@ constexpr int kQ = 4;                   // columns per lane
 constexpr int kStrip = kQ * kLanes;     // 256 columns per wave
 constexpr int kHalo = 12;               // columns of halo each side
 constexpr int kOut = kStrip - 2 * kHalo; // 232 output columns per strip
-constexpr int kRing = 17;               // raw rows held: entering .. 16 above
-constexpr uint32_t kStage = 128;        // signal pixels staged per wave
+constexpr int kRing = 16;               // raw rows held: entering .. 15 above
+constexpr uint32_t kStage = 48;         // candidates staged per wave
 constexpr uint32_t kFast = 1u << 18;    // fast-path bound on valid pixels
+constexpr uint32_t kWide = 1u << 16;    // pixels at or above this use the byte ring
+constexpr int kTable = 64;              // root table entries used (m <= 49)
 constexpr uint32_t kCount0 = 1u << 26;  // count unit in stage 0's word
 constexpr uint32_t kCount2 = 1u << 25;  // count unit in stage 2's word
 
@@ -55,14 +78,16 @@ using sfi::FusedParams;
 
 // Lane shifts across the whole wave.  DPP wave_shr:1 gives lane i the value of
 // lane i-1 (lane 0 gets 0); wave_shl:1 gives lane i the value of lane i+1
-// (lane 63 gets 0).  The edge lanes' results are halo and never used.
+// (lane 63 gets 0): bound_ctrl makes a source lane outside the wave read as
+// zero, so no old value is needed and an add can take the shift as its
+// operand.  The edge lanes' results are halo and never used.
 __device__ __forceinline__ uint32_t from_left(uint32_t v) {
   return static_cast<uint32_t>(
-      __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x138, 0xf, 0xf, false));
+      __builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x138, 0xf, 0xf, true));
 }
 __device__ __forceinline__ uint32_t from_right(uint32_t v) {
   return static_cast<uint32_t>(
-      __builtin_amdgcn_update_dpp(0, static_cast<int>(v), 0x130, 0xf, 0xf, false));
+      __builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x130, 0xf, 0xf, true));
 }
 __device__ __forceinline__ double from_left(double v) {
   const uint64_t b = __double_as_longlong(v);
@@ -79,82 +104,115 @@ __device__ __forceinline__ double from_right(double v) {
       (static_cast<uint64_t>(hi) << 32) | lo));
 }
 
-template <typename T> struct Row {
-  uint32_t v[kQ];
-};
+// A lane's four pixels as one load; dword alignment is all the hardware
+// asks of a multi-dword global load.
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4), aligned(4)));
+typedef uint16_t u16x4 __attribute__((ext_vector_type(4), aligned(2)));
+typedef uint32_t u32x4a __attribute__((ext_vector_type(4))); // 16-byte aligned
 
-// Loads this lane's four pixels of row y; outside the frame they are masked.
+// Loads this lane's four pixels of in-frame row y; columns outside the frame
+// read as 0 (their masked bits are constant per lane).
 template <typename T>
-__device__ __forceinline__ void load_row(const T *__restrict__ frame, int height,
+__device__ __forceinline__ void load_row(const SF_GLOBAL T *__restrict__ frame,
                                          int width, int y, int c0,
                                          uint32_t (&v)[kQ]) {
-  const uint32_t masked = traits<T>::masked;
-  if (y < 0 || y >= height) {
-#pragma unroll
-    for (int q = 0; q < kQ; q++)
-      v[q] = masked;
+  const SF_GLOBAL T *row = frame + static_cast<size_t>(y) * width;
+  if (c0 >= 0 && c0 + kQ <= width) {
+    if constexpr (sizeof(T) == 4) {
+      const u32x4 x = *reinterpret_cast<const SF_GLOBAL u32x4 *>(row + c0);
+      v[0] = x[0], v[1] = x[1], v[2] = x[2], v[3] = x[3];
+    } else {
+      const u16x4 x = *reinterpret_cast<const SF_GLOBAL u16x4 *>(row + c0);
+      v[0] = x[0], v[1] = x[1], v[2] = x[2], v[3] = x[3];
+    }
     return;
   }
-  const T *row = frame + static_cast<size_t>(y) * width;
 #pragma unroll
   for (int q = 0; q < kQ; q++) {
     const int c = c0 + q;
-    v[q] = (c >= 0 && c < width) ? static_cast<uint32_t>(row[c]) : masked;
+    v[q] = (c >= 0 && c < width) ? static_cast<uint32_t>(row[c]) : 0u;
   }
 }
 
-template <typename T>
-__device__ __forceinline__ uint32_t pack0(uint32_t v) {
-  return v < traits<T>::masked ? v + kCount0 : 0u;
+// The byte ring's form of a lane's four pixels (each below 2^18): bits 16-17
+// of each.
+__device__ __forceinline__ uint32_t pack_high(const uint32_t (&v)[kQ]) {
+  return (v[0] >> 16) | ((v[1] >> 16) << 2) | ((v[2] >> 16) << 4) |
+         ((v[3] >> 16) << 6);
 }
-template <typename T> __device__ __forceinline__ double square(uint32_t v) {
-  const double p = v < traits<T>::masked ? static_cast<double>(v) : 0.0;
-  return p * p;
-}
-template <typename T>
-__device__ __forceinline__ uint32_t pack2(uint32_t v, uint32_t eroded) {
-  return (v < traits<T>::masked && !eroded) ? v + kCount2 : 0u;
+__device__ __forceinline__ void add_high(uint32_t h, uint32_t (&v)[kQ]) {
+#pragma unroll
+  for (int q = 0; q < kQ; q++)
+    v[q] |= ((h >> (2 * q)) & 3u) << 16;
 }
 
-// Stage 0's test on the fast path's exact sums, in the reference's real type.
+// Sums over 7 columns centred on each of a lane's four, c-3 .. c+3, with the
+// neighbouring lanes' columns by lane shifts.
+template <typename V>
+__device__ __forceinline__ void across7(const V (&v)[kQ], V (&h)[kQ]) {
+  const V l1 = from_left(v[1]), l2 = from_left(v[2]), l3 = from_left(v[3]);
+  const V r0 = from_right(v[0]), r1 = from_right(v[1]), r2 = from_right(v[2]);
+  h[0] = ((l1 + l2) + (l3 + v[0])) + ((v[1] + v[2]) + v[3]);
+  h[1] = (h[0] - l1) + r0;
+  h[2] = (h[1] - l2) + r1;
+  h[3] = (h[2] - l3) + r2;
+}
+
+// Stage 0's test (dext.cc:99-110) on exact window sums: `count` valid pixels
+// summing to `sum`, squares summing to `sum_sq`; the table holds
+// R(sqrt(2 (m - 1))), +inf below two, so that m < 2 tests false.
 template <typename T>
-__device__ __forceinline__ bool stage0_test(uint32_t word, double sum_sq,
-                                            const typename traits<T>::R *table);
+__device__ __forceinline__ bool stage0_test(uint32_t count, uint32_t sum,
+                                            double sum_sq,
+                                            typename traits<T>::R root);
 
 template <>
-__device__ __forceinline__ bool stage0_test<uint32_t>(uint32_t word,
+__device__ __forceinline__ bool stage0_test<uint32_t>(uint32_t count,
+                                                      uint32_t sum,
                                                       double sum_sq,
-                                                      const double *table) {
-  const uint32_t count = word >> 26;
+                                                      double root) {
   const double m = static_cast<double>(count);
-  const double s = static_cast<double>(word & (kCount0 - 1));
-  const double lhs = (m * sum_sq - s * s) - s * (m - 1.0);
-  const double rhs = (s * 6.0) * table[count]; // +inf below two: false
+  const double s = static_cast<double>(sum);
+  // (m * s2 - s * s) - s * (m - 1): every term and partial result is an
+  // integer below 2^53, so this one-rounding form is the same exact value.
+  const double lhs = __fma_rn(m, sum_sq, -(s * (s + (m - 1.0))));
+  const double rhs = (s * 6.0) * root;
   return lhs > rhs;
 }
 
 template <>
-__device__ __forceinline__ bool stage0_test<uint16_t>(uint32_t word,
+__device__ __forceinline__ bool stage0_test<uint16_t>(uint32_t count,
+                                                      uint32_t sum,
                                                       double sum_sq,
-                                                      const float *table) {
-  const uint32_t count = word >> 26;
+                                                      float root) {
   const float m = static_cast<float>(count);
-  const float s = static_cast<float>(word & (kCount0 - 1));
+  const float s = static_cast<float>(sum);
   // The reference's accumulator is uint32 and wraps; the exact sum here is
   // below 2^38, so its low 32 bits are the wrapped value.
   const uint32_t wrapped =
       static_cast<uint32_t>(static_cast<uint64_t>(sum_sq));
   const float s2 = static_cast<float>(wrapped);
   const float lhs = (m * s2 - s * s) - s * (m - 1.0f);
-  const float rhs = (s * 6.0f) * table[count];
+  const float rhs = (s * 6.0f) * root;
   return lhs > rhs;
 }
 
-// What one wave keeps in LDS: the ring of raw rows and the signal staging.
+// What one wave keeps in LDS: the rings of raw rows (the low ring column by
+// column, so that a lane's 16-bit reads fall in distinct banks) and the
+// staged stage-2 candidates.
 struct WaveLds {
-  uint4 ring[kRing][kLanes];
-  sf_signal_pixel stage[kStage];
+  uint16_t ring[kRing][kQ][kLanes];
+  uint8_t high[kRing][kLanes];
+  uint4 stage[kStage];
 };
+
+// Bits of column q in rows of a nibble history: nibbles k0 .. k0 + n - 1.
+__host__ __device__ constexpr uint64_t column_bits(int k0, int n, int q) {
+  uint64_t m = 0;
+  for (int k = k0; k < k0 + n; k++)
+    m |= 1ull << (4 * k + q);
+  return m;
+}
 
 // One wave's sweep of one strip segment of one frame: `item` numbers the
 // frame's segments strip-fastest.
@@ -171,142 +229,267 @@ __device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
   const int c0 = x0 - kHalo + kQ * static_cast<int>(lane); // first column
   const int y0 = seg * static_cast<int>(a.seg_rows);
   const int y1 = min(height, y0 + static_cast<int>(a.seg_rows));
-  const T *frame = static_cast<const T *>(a.frame);
-  uint4(*ring)[kLanes] = w.ring;
+  const SF_GLOBAL T *frame = gmem(static_cast<const T *>(a.frame));
+  // records move as 16-byte vectors
+  SF_GLOBAL u32x4a *out = reinterpret_cast<SF_GLOBAL u32x4a *>(gmem(a.out));
+  SF_GLOBAL uint32_t *mask = gmem(a.mask);
+  uint16_t(*ring)[kQ][kLanes] = w.ring;
+  uint8_t(*high)[kLanes] = w.high;
+  auto ring_row = [&](int s, uint32_t(&v)[kQ]) {
+#pragma unroll
+    for (int q = 0; q < kQ; q++)
+      v[q] = ring[s][q][lane];
+  };
   const uint32_t masked = traits<T>::masked;
+  // A row whose pixels are all below this holds no masked pixel and none
+  // that needs the byte ring or the exact path.
+  const uint32_t plain = sizeof(T) == 4 ? kWide : masked;
+  const R root49 = table[49];
 
-  // Signal pixels wait in LDS and go out kStage at a time, one atomic on the
-  // frame's counter per flush: a returning atomic per row on that one word
-  // serialises every wave of the device behind it.
-  sf_signal_pixel *stage = w.stage;
+  // Stage-2 candidates wait in LDS, kStage (one a lane) at a time, each as
+  // {index, value | mask bit << 18, count | sum word, mask word}.  A flush
+  // tests them one a lane, so the test's divide and square root run at full
+  // width, and appends the signal pixels with one atomic on the frame's
+  // counter: a returning atomic per row on that one word serialises every
+  // wave of the device behind it.
+  uint4 *stage = w.stage;
   uint32_t staged = 0; // wave-uniform
   auto flush = [&]() {
     if (staged == 0)
       return;
-    uint32_t base = 0;
-    if (lane == 0)
-      base = atomicAdd(a.counter, staged);
-    base = __shfl(base, 0);
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-    for (uint32_t i = lane; i < staged; i += kLanes)
-      if (base + i < a.capacity)
-        a.out[base + i] = stage[i];
+    bool signal = false;
+    u32x4a rec = {0u, 0u, 0u, 0u};
+    uint32_t mword = 0, mbit = 0;
+    if (lane < staged) {
+      const uint4 c = stage[lane];
+      const uint32_t p = c.y & (kFast - 1);
+      const uint32_t m2 = c.z >> 25;
+      R mean;
+      signal = poisson<Wt, R>(p, static_cast<Wt>(m2),
+                              static_cast<Wt>(c.z & (kCount2 - 1)), &mean);
+      // {index, value, background, population | reserved 0}
+      rec = u32x4a{c.x, p, __float_as_uint(static_cast<float>(mean)), m2};
+      mword = c.w;
+      mbit = c.y >> 18;
+    }
+    const uint64_t b = __ballot(signal);
+    if (b != 0) {
+      uint32_t base = 0;
+      if (lane == 0)
+        base = __hip_atomic_fetch_add(gmem(a.counter),
+                                      static_cast<uint32_t>(__popcll(b)),
+                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+      base = __shfl(base, 0);
+      if (signal) {
+        const uint32_t i =
+            base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)));
+        if (i < a.capacity)
+          out[i] = rec;
+        __hip_atomic_fetch_or(mask + mword, 1u << mbit, __ATOMIC_RELAXED,
+                              __HIP_MEMORY_SCOPE_AGENT);
+      }
+    }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
     staged = 0;
   };
-  auto put = [&](bool signal, const sf_signal_pixel &rec) {
-    const uint64_t b = __ballot(signal);
+  // One ballot can hold more candidates (up to a wave) than the stage has
+  // room for, so they go in chunks that fit, flushing in between: `staged`
+  // never exceeds kStage, and a flush reads only entries written since the
+  // last one.
+  auto put = [&](bool cand, const uint4 &entry) {
+    const uint64_t b = __ballot(cand);
     if (b == 0)
       return;
     const uint32_t n = static_cast<uint32_t>(__popcll(b));
-    if (staged + n > kStage)
-      flush();
-    if (signal)
-      stage[staged + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)))] =
-          rec;
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    staged += n;
+    const uint32_t rank =
+        static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)));
+    for (uint32_t done = 0; done < n;) {
+      if (staged == kStage)
+        flush();
+      const uint32_t take = min(kStage - staged, n - done);
+      if (cand && rank >= done && rank < done + take)
+        stage[staged + (rank - done)] = entry;
+      __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+      __builtin_amdgcn_wave_barrier();
+      staged += take;
+      done += take;
+    }
   };
-  bool saw_big = false;
+  bool big = false; // a valid pixel of this lane at or above kFast
 
-  // Output columns of this lane inside the strip's middle and the frame.
-  uint32_t in_out = 0;
+  // Columns of this lane outside the frame (masked in every row), and those
+  // it outputs: inside the strip's middle and the frame.
+  uint32_t outside = 0, in_out = 0;
 #pragma unroll
   for (int q = 0; q < kQ; q++) {
     const int c = c0 + q;
-    if (c >= x0 && c < x0 + kOut && c < width)
+    if (c < 0 || c >= width)
+      outside |= 1u << q;
+    else if (c >= x0 && c < x0 + kOut)
       in_out |= 1u << q;
   }
 
-  uint32_t v0w[kQ] = {0, 0, 0, 0};   // stage 0 vertical: count | sum
+  // The ring starts at zero: rows before the first add and remove nothing.
+  for (int s = 0; s < kRing; s++)
+#pragma unroll
+    for (int q = 0; q < kQ; q++)
+      ring[s][q][lane] = 0;
+
+  uint32_t v0w[kQ] = {0, 0, 0, 0};       // stage 0 vertical: sum
   double v0s[kQ] = {0.0, 0.0, 0.0, 0.0}; // stage 0 vertical: sum of squares
-  uint32_t v2w[kQ] = {0, 0, 0, 0};   // stage 2 vertical: count | sum
-  uint32_t mhist = 0;  // masked nibbles, row y_in - k at nibble k
-  uint32_t hreg = 0;   // horizontal-erode nibbles, row yc - k at nibble k
-  uint64_t s1hist = 0; // scr1 nibbles, row ye - k at nibble k
+  uint32_t v2w[kQ] = {0, 0, 0, 0};       // stage 2 vertical: sum
+  // Masked nibbles, row y_in - k at nibble k; rows before the first count as
+  // masked (their ring values are zero).
+  uint64_t mhist = ~0ull;
+  // Unmasked scr1 nibbles, row (y_in - 5) - k at nibble k.
+  uint64_t x1hist = 0;
+  uint32_t hreg = 0;      // horizontal-erode nibbles, row y_in-3 - k at nibble k
+  uint32_t wide_rows = 0; // wave-uniform; row y_in - k at bit k once y_in is in
 
   const int steps = (y1 - y0) + 20;
-  uint32_t next[kQ];
-  load_row<T>(frame, height, width, y0 - 10, c0, next);
+  // Two rows in flight from HBM: the buffer a row is read from takes the load
+  // of the row two below as soon as its values are in registers.
+  uint32_t ahead0[kQ] = {0, 0, 0, 0}, ahead1[kQ] = {0, 0, 0, 0};
+  if (y0 - 10 >= 0 && y0 - 10 < height)
+    load_row<T>(frame, width, y0 - 10, c0, ahead0);
+  if (y0 - 9 >= 0 && y0 - 9 < height)
+    load_row<T>(frame, width, y0 - 9, c0, ahead1);
   int slot = 0; // ring slot of the entering row
-  for (int t = 0; t < steps; t++) {
+  auto row = [&](int t, uint32_t(&buf)[kQ]) {
     const int y_in = y0 - 10 + t;
-    uint32_t cur[kQ];
-#pragma unroll
-    for (int q = 0; q < kQ; q++)
-      cur[q] = next[q];
-    if (t + 1 < steps)
-      load_row<T>(frame, height, width, y_in + 1, c0, next);
 
-    // Fast-path guard: a valid pixel at or above 2^18 voids this frame's
-    // fast sums; the caller re-runs it exactly.
-    bool big = false;
-#pragma unroll
-    for (int q = 0; q < kQ; q++)
-      big |= cur[q] < masked && cur[q] >= kFast;
-    saw_big = saw_big || __ballot(big) != 0;
-
-    ring[slot][lane] = make_uint4(cur[0], cur[1], cur[2], cur[3]);
-    uint32_t mnib = 0;
-#pragma unroll
-    for (int q = 0; q < kQ; q++)
-      mnib |= (cur[q] >= masked ? 1u : 0u) << q;
-    mhist = (mhist << 4) | mnib;
-
-    auto at = [&](int back) {
-      int s = slot - back;
-      if (s < 0)
-        s += kRing;
-      return ring[s][lane];
-    };
-
-    // ---- stage 0: vertical 7-row sums (rows y_in-6 .. y_in) ----
-    {
-      uint32_t out[kQ] = {masked, masked, masked, masked};
-      if (t >= 7) {
-        const uint4 o = at(7);
-        out[0] = o.x, out[1] = o.y, out[2] = o.z, out[3] = o.w;
-      }
-#pragma unroll
-      for (int q = 0; q < kQ; q++) {
-        v0w[q] += pack0<T>(cur[q]) - pack0<T>(out[q]);
-        v0s[q] += square<T>(cur[q]) - square<T>(out[q]);
-      }
-    }
-
-    // ---- stage 0: 7 columns across lanes, test -> scr0 of row y_in-3 ----
-    uint32_t scr0 = 0;
-    {
-      const uint32_t lw1 = from_left(v0w[1]), lw2 = from_left(v0w[2]),
-                     lw3 = from_left(v0w[3]);
-      const uint32_t rw0 = from_right(v0w[0]), rw1 = from_right(v0w[1]),
-                     rw2 = from_right(v0w[2]);
-      const double ls1 = from_left(v0s[1]), ls2 = from_left(v0s[2]),
-                   ls3 = from_left(v0s[3]);
-      const double rs0 = from_right(v0s[0]), rs1 = from_right(v0s[1]),
-                   rs2 = from_right(v0s[2]);
-      uint32_t hw[kQ];
-      double hs[kQ];
-      hw[0] = lw1 + lw2 + lw3 + v0w[0] + v0w[1] + v0w[2] + v0w[3];
-      hw[1] = hw[0] - lw1 + rw0;
-      hw[2] = hw[1] - lw2 + rw1;
-      hw[3] = hw[2] - lw3 + rw2;
-      hs[0] = ((ls1 + ls2) + (ls3 + v0s[0])) + ((v0s[1] + v0s[2]) + v0s[3]);
-      hs[1] = (hs[0] - ls1) + rs0;
-      hs[2] = (hs[1] - ls2) + rs1;
-      hs[3] = (hs[2] - ls3) + rs2;
-      const uint32_t m3 = (mhist >> 12) & 0xFu; // masked nibble of row y_in-3
+    // ---- the entering row: values with masked as zero, masked bits ----
+    uint32_t val[kQ];
+    uint32_t mnib = outside;
+    bool wide = false;
+    if (y_in < 0 || y_in >= height) {
 #pragma unroll
       for (int q = 0; q < kQ; q++)
-        if (stage0_test<T>(hw[q], hs[q], table))
-          scr0 |= 1u << q;
-      scr0 |= m3;
+        val[q] = 0;
+      mnib = 0xFu;
+    } else {
+#pragma unroll
+      for (int q = 0; q < kQ; q++)
+        val[q] = buf[q];
+      const uint32_t top = max(max(val[0], val[1]), max(val[2], val[3]));
+      if (__builtin_expect(__ballot(top >= plain) != 0, 0)) {
+        // Masked pixels become zero; a valid pixel at or above kFast voids
+        // this frame's fast sums (the caller re-runs it exactly) and is
+        // clamped so that no sum spills into its count bits.
+#pragma unroll
+        for (int q = 0; q < kQ; q++) {
+          const bool m = val[q] >= masked;
+          mnib |= (m ? 1u : 0u) << q;
+          big |= !m && val[q] >= kFast;
+          val[q] = m ? 0u : min(val[q], kFast - 1u);
+        }
+        wide = __ballot((val[0] | val[1] | val[2] | val[3]) >= kWide) != 0;
+      }
+    }
+    if (t + 2 < steps && y_in + 2 >= 0 && y_in + 2 < height)
+      load_row<T>(frame, width, y_in + 2, c0, buf);
+
+    // Ring rows leaving stage 0 (y_in - 7), entering stage 2 (y_in - 5) and
+    // leaving it (y_in - 16, whose slot the entering row takes once it has
+    // been read), low 16 bits.
+    const int s7 = (slot - 7) & (kRing - 1);
+    const int s5 = (slot - 5) & (kRing - 1);
+    const int s16 = slot;
+    // (Read in the same block as their first use, so that the 16-bit loads'
+    // zero extension folds into them.)
+    uint32_t out0[kQ], in2[kQ], out2[kQ];
+    ring_row(s7, out0);
+    ring_row(s5, in2);
+    ring_row(s16, out2);
+#pragma unroll
+    for (int q = 0; q < kQ; q++)
+      ring[slot][q][lane] = static_cast<uint16_t>(val[q]);
+    mhist = (mhist << 4) | mnib;
+
+    // ---- stage 0: vertical 7-row sums (rows y_in-6 .. y_in) ----
+#pragma unroll
+    for (int q = 0; q < kQ; q++) {
+      const uint32_t d = val[q] - out0[q];
+      v0w[q] += d;
+      v0s[q] = __fma_rn(static_cast<double>(static_cast<int32_t>(d)),
+                        static_cast<double>(val[q] + out0[q]), v0s[q]);
+    }
+    // Stage 2's entering and leaving rows, as a change to its sums.
+    uint32_t d2[kQ];
+#pragma unroll
+    for (int q = 0; q < kQ; q++)
+      d2[q] = in2[q] - out2[q];
+
+    // The ring rows above hold only the low 16 bits.  For the rare rows with
+    // a pixel at or above 2^16 (before row y_in is in, row y_in - k is bit
+    // k - 1 of wide_rows), add each such pixel's high part H to what used
+    // it: stage 0's sums lose H more and their squares (o + H)^2 - o^2 more;
+    // stage 2's entering and leaving values gain H.
+    uint32_t in2h[kQ] = {0, 0, 0, 0}, out2h[kQ] = {0, 0, 0, 0};
+    if (__builtin_expect((wide_rows & ((1u << 6) | (1u << 4) | (1u << 15))) != 0,
+                         0)) {
+      if ((wide_rows >> 6) & 1u) {
+        uint32_t h[kQ] = {0, 0, 0, 0};
+        add_high(high[s7][lane], h);
+#pragma unroll
+        for (int q = 0; q < kQ; q++) {
+          v0w[q] -= h[q];
+          v0s[q] -= static_cast<double>(h[q]) *
+                    static_cast<double>(2u * out0[q] + h[q]);
+        }
+      }
+      if ((wide_rows >> 4) & 1u)
+        add_high(high[s5][lane], in2h);
+      if ((wide_rows >> 15) & 1u)
+        add_high(high[s16][lane], out2h);
+#pragma unroll
+      for (int q = 0; q < kQ; q++) {
+        in2[q] += in2h[q];
+        out2[q] += out2h[q];
+        d2[q] = in2[q] - out2[q];
+      }
+    }
+    wide_rows = (wide_rows << 1) | (wide ? 1u : 0u);
+    if (__builtin_expect(wide, 0))
+      high[slot][lane] = static_cast<uint8_t>(pack_high(val));
+
+    // ---- stage 0: 7 columns across lanes, test -> scr0 of row y_in-3 ----
+    uint32_t scr0 = static_cast<uint32_t>(mhist >> 12) & 0xFu; // masked, y_in-3
+    {
+      double hs[kQ];
+      across7(v0s, hs);
+      // A masked pixel in rows y_in-6 .. y_in of any lane: counts vary and
+      // travel in the top bits of the sums; otherwise every count is 49.
+      const uint32_t m7 = static_cast<uint32_t>(mhist) & 0x0FFFFFFFu;
+      if (__ballot(m7 != 0) == 0) {
+        uint32_t hw[kQ];
+        across7(v0w, hw);
+#pragma unroll
+        for (int q = 0; q < kQ; q++)
+          if (stage0_test<T>(49u, hw[q], hs[q], root49))
+            scr0 |= 1u << q;
+      } else {
+        uint32_t word[kQ], hw[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; q++)
+          word[q] = v0w[q] +
+                    ((7u - static_cast<uint32_t>(__popc(
+                               m7 & static_cast<uint32_t>(column_bits(0, 7, q)))))
+                     << 26);
+        across7(word, hw);
+#pragma unroll
+        for (int q = 0; q < kQ; q++) {
+          const uint32_t m = hw[q] >> 26;
+          if (stage0_test<T>(m, hw[q] & (kCount0 - 1), hs[q], table[m]))
+            scr0 |= 1u << q;
+        }
+      }
     }
 
     // ---- stage 1: erode 5 x 5 -> scr1 of row y_in-5 ----
-    uint32_t scr1;
+    uint32_t x1; // scr1 and not masked
     {
       const uint32_t left = from_left(scr0) & 0xFu;
       const uint32_t right = from_right(scr0) & 0xFu;
@@ -316,101 +499,112 @@ __device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
       hreg = (hreg << 4) | ((y >> 2) & 0xFu);
       const uint32_t all5 =
           hreg & (hreg >> 4) & (hreg >> 8) & (hreg >> 12) & (hreg >> 16) & 0xFu;
-      const uint32_t m5 = (mhist >> 20) & 0xFu; // masked nibble of row y_in-5
-      scr1 = all5 | m5;
-      s1hist = (s1hist << 4) | scr1;
+      // scr1 = all5 | masked; a masked pixel adds zero to every sum, so only
+      // the unmasked part of scr1 needs excluding.
+      x1 = all5 & ~(static_cast<uint32_t>(mhist >> 20) & 0xFu);
     }
 
-    // ---- stage 2: vertical 11-row sums of valid, not-scr1 pixels ----
-    if (t >= 5) {
-      const uint4 e = at(5); // row y_in-5, whose scr1 is nibble 0
-      v2w[0] += pack2<T>(e.x, scr1 & 1u);
-      v2w[1] += pack2<T>(e.y, scr1 & 2u);
-      v2w[2] += pack2<T>(e.z, scr1 & 4u);
-      v2w[3] += pack2<T>(e.w, scr1 & 8u);
-    }
-    if (t >= 16) {
-      const uint4 o = at(16); // row y_in-16, whose scr1 is nibble 11
-      const uint32_t s = static_cast<uint32_t>(s1hist >> 44) & 0xFu;
-      v2w[0] -= pack2<T>(o.x, s & 1u);
-      v2w[1] -= pack2<T>(o.y, s & 2u);
-      v2w[2] -= pack2<T>(o.z, s & 4u);
-      v2w[3] -= pack2<T>(o.w, s & 8u);
+    // ---- stage 2: vertical 11-row sums of values outside scr1 ----
+    {
+      const uint32_t xout = static_cast<uint32_t>(x1hist >> 40) & 0xFu; // y_in-16
+      if (__ballot((x1 | xout) != 0) == 0) {
+#pragma unroll
+        for (int q = 0; q < kQ; q++)
+          v2w[q] += d2[q];
+      } else {
+#pragma unroll
+        for (int q = 0; q < kQ; q++)
+          v2w[q] += (((x1 >> q) & 1u) ? 0u : in2[q]) -
+                    (((xout >> q) & 1u) ? 0u : out2[q]);
+      }
+      x1hist = (x1hist << 4) | x1;
     }
 
-    // ---- stage 2: 11 columns across lanes, candidates of row y_in-10 ----
+    // ---- stage 2: candidates of row y_in-10 ----
     if (t >= 20) {
       const int y2 = y_in - 10;
-      const uint32_t s1 = static_cast<uint32_t>(s1hist >> 20) & 0xFu;
-      const uint32_t maybe = s1 & in_out;
+      const uint32_t maybe = static_cast<uint32_t>(x1hist >> 20) & in_out;
       if (__ballot(maybe != 0) != 0) {
-        const uint32_t l0 = from_left(v2w[0]), l1 = from_left(v2w[1]),
-                       l2 = from_left(v2w[2]), l3 = from_left(v2w[3]);
-        const uint32_t r0 = from_right(v2w[0]), r1 = from_right(v2w[1]),
-                       r2 = from_right(v2w[2]), r3 = from_right(v2w[3]);
+        // Counts: rows y_in-15 .. y_in-5, less masked and unmasked-scr1.
+        uint32_t word[kQ];
+#pragma unroll
+        for (int q = 0; q < kQ; q++) {
+          const uint32_t excluded =
+              static_cast<uint32_t>(__popcll(mhist & column_bits(5, 11, q))) +
+              static_cast<uint32_t>(__popcll(x1hist & column_bits(0, 11, q)));
+          word[q] = v2w[q] + ((11u - excluded) << 25);
+        }
+        const uint32_t l0 = from_left(word[0]), l1 = from_left(word[1]),
+                       l2 = from_left(word[2]), l3 = from_left(word[3]);
+        const uint32_t r0 = from_right(word[0]), r1 = from_right(word[1]),
+                       r2 = from_right(word[2]), r3 = from_right(word[3]);
         const uint32_t ll3 = from_left(l3), rr0 = from_right(r0);
         uint32_t h[kQ];
-        h[0] = ll3 + l0 + l1 + l2 + l3 + v2w[0] + v2w[1] + v2w[2] + v2w[3] +
+        h[0] = ll3 + l0 + l1 + l2 + l3 + word[0] + word[1] + word[2] + word[3] +
                r0 + r1;
         h[1] = h[0] - ll3 + r2;
         h[2] = h[1] - l0 + r3;
         h[3] = h[2] - l1 + rr0;
-        const uint4 pv = at(10);
-        const uint32_t raw[kQ] = {pv.x, pv.y, pv.z, pv.w};
+        const int s10 = (slot - 10) & (kRing - 1);
+        uint32_t raw[kQ];
+        ring_row(s10, raw);
+        if ((wide_rows >> 10) & 1u)
+          add_high(high[s10][lane], raw);
 #pragma unroll
         for (int q = 0; q < kQ; q++) {
-          const uint32_t p = raw[q] >= masked ? 0u : raw[q];
-          bool signal = false;
-          sf_signal_pixel rec{};
-          if (((maybe >> q) & 1u) && p > 0) {
-            const uint32_t m2 = h[q] >> 25;
-            const uint32_t s2 = h[q] & (kCount2 - 1);
-            R mean;
-            signal = poisson<Wt, R>(p, static_cast<Wt>(m2), static_cast<Wt>(s2),
-                                    &mean);
-            const int c = c0 + q;
-            rec.index = static_cast<uint32_t>(y2) * a.width +
-                        static_cast<uint32_t>(c);
-            rec.value = p;
-            rec.background = static_cast<float>(mean);
-            rec.population = static_cast<uint16_t>(m2);
-            rec.reserved = 0;
-            if (signal)
-              atomicOr(&a.mask[static_cast<size_t>(y2) * a.wpr + c / 32],
-                       1u << (c & 31));
-          }
-          put(signal, rec);
+          const int c = c0 + q;
+          const bool cand = ((maybe >> q) & 1u) && raw[q] > 0;
+          const uint4 entry = make_uint4(
+              static_cast<uint32_t>(y2) * a.width + static_cast<uint32_t>(c),
+              raw[q] | (static_cast<uint32_t>(c & 31) << 18), h[q],
+              static_cast<uint32_t>(y2) * a.wpr + static_cast<uint32_t>(c) / 32);
+          put(cand, entry);
         }
       }
     }
-    slot = slot + 1 == kRing ? 0 : slot + 1;
+    slot = (slot + 1) & (kRing - 1);
+  };
+  int t = 0;
+  for (; t + 1 < steps; t += 2) {
+    row(t, ahead0);
+    row(t + 1, ahead1);
   }
+  if (t < steps)
+    row(t, ahead0);
   flush();
-  if (saw_big && lane == 0)
-    atomicOr(a.overflow, 1u);
+  if (__ballot(big) != 0 && lane == 0)
+    __hip_atomic_fetch_or(gmem(a.overflow), 1u, __ATOMIC_RELAXED,
+                          __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Every wave takes items in turn, grid-stride, over all the frames of a
-// batch (item_base ascending), any mix of sizes and pixel types in one
-// launch.  The grid is sized to stay resident, so the launch is persistent
-// and every wave leaves when the items run out.
+// Every wave takes items from a queue (one atomic a wave per item), over all
+// the frames of a batch (item_base ascending), any mix of sizes and pixel
+// types in one launch: items of unequal cost then finish within about one
+// item of each other.  The grid is the device's resident waves, so the launch
+// is persistent, and every wave leaves when the queue runs past the items.
 template <int Waves>
 __global__ __launch_bounds__(Waves *kLanes) void fused_kernel(
     const FusedParams *__restrict__ frames, uint32_t nframes,
     uint32_t total_items, const float *__restrict__ tf,
-    const double *__restrict__ td) {
+    const double *__restrict__ td, uint32_t *__restrict__ queue) {
   __shared__ WaveLds lds[Waves];
-  __shared__ float table_f[128];
-  __shared__ double table_d[128];
-  for (int i = threadIdx.x; i < 128; i += blockDim.x) {
+  __shared__ float table_f[kTable];
+  __shared__ double table_d[kTable];
+  for (int i = threadIdx.x; i < kTable; i += blockDim.x) {
     table_f[i] = tf[i];
     table_d[i] = td[i];
   }
   __syncthreads();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
   const uint32_t lane = threadIdx.x % kLanes;
-  for (uint32_t item = blockIdx.x * Waves + wave; item < total_items;
-       item += gridDim.x * Waves) {
+  for (;;) {
+    uint32_t item = 0;
+    if (lane == 0)
+      item = __hip_atomic_fetch_add(gmem(queue), 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    item = __builtin_amdgcn_readfirstlane(item);
+    if (item >= total_items)
+      break;
     uint32_t lo = 0, hi = nframes;
     while (hi - lo > 1) {
       const uint32_t mid = (lo + hi) / 2;
@@ -428,27 +622,28 @@ __global__ __launch_bounds__(Waves *kLanes) void fused_kernel(
   }
 }
 
-void launch(const FusedParams *frames, uint32_t nframes, uint32_t total_items,
-            const float *tf, const double *td, int waves_per_block,
-            uint32_t max_waves, hipStream_t stream) {
-  if (total_items == 0)
-    return;
-  const uint32_t waves = total_items < max_waves ? total_items : max_waves;
-  switch (waves_per_block) {
-  case 1:
-    fused_kernel<1><<<waves, kLanes, 0, stream>>>(frames, nframes, total_items,
-                                                  tf, td);
-    break;
-  case 4:
-    fused_kernel<4><<<(waves + 3) / 4, 4 * kLanes, 0, stream>>>(
-        frames, nframes, total_items, tf, td);
-    break;
-  default:
-    fused_kernel<2><<<(waves + 1) / 2, 2 * kLanes, 0, stream>>>(
-        frames, nframes, total_items, tf, td);
-    break;
-  }
-  sfi::check(hipGetLastError(), "launching the fused threshold");
+// Resident blocks of a kernel variant on one CU, from the occupancy the
+// compiler's register and LDS use allow; computed once.
+template <int Waves> uint32_t resident_blocks() {
+  static const uint32_t blocks = [] {
+    int n = 0;
+    sfi::check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
+                   &n, fused_kernel<Waves>, Waves * kLanes, 0),
+               "querying the fused kernel's occupancy");
+    return static_cast<uint32_t>(n > 0 ? n : 1);
+  }();
+  return blocks;
+}
+
+template <int Waves>
+void launch_waves(const FusedParams *frames, uint32_t nframes,
+                  uint32_t total_items, const float *tf, const double *td,
+                  uint32_t *queue, uint32_t cus, hipStream_t stream) {
+  const uint32_t resident = cus * resident_blocks<Waves>();
+  const uint32_t need = (total_items + Waves - 1) / Waves;
+  const uint32_t blocks = need < resident ? need : resident;
+  fused_kernel<Waves><<<blocks, Waves * kLanes, 0, stream>>>(
+      frames, nframes, total_items, tf, td, queue);
 }
 
 } // namespace
@@ -479,18 +674,28 @@ uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
 
 void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
                         uint32_t total_items, const float *tf, const double *td,
-                        hipStream_t stream) {
+                        uint32_t *queue, hipStream_t stream) {
+  if (total_items == 0)
+    return;
   const Tuning t = tuning();
-  // Resident waves: sized so that every CU holds several; the items are
-  // spread over them grid-stride.
   int cus = 256;
   int device = 0;
   if (hipGetDevice(&device) == hipSuccess)
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                 device);
-  const uint32_t max_waves = static_cast<uint32_t>(cus) * 16u;
-  launch(frames_device, nframes, total_items, tf, td, t.waves_per_block,
-         max_waves, stream);
+  const uint32_t n = static_cast<uint32_t>(cus);
+  switch (t.waves_per_block) {
+  case 1:
+    launch_waves<1>(frames_device, nframes, total_items, tf, td, queue, n, stream);
+    break;
+  case 4:
+    launch_waves<4>(frames_device, nframes, total_items, tf, td, queue, n, stream);
+    break;
+  default:
+    launch_waves<2>(frames_device, nframes, total_items, tf, td, queue, n, stream);
+    break;
+  }
+  check(hipGetLastError(), "launching the fused threshold");
 }
 
 FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,

@@ -119,6 +119,7 @@ def lib():
         "sf_device_free": ([vp], None),
         "sf_copy_to_device": ([vp, vp, sz], ctypes.c_int),
         "sf_copy_to_host": ([vp, vp, sz], ctypes.c_int),
+        "sf_copy_on_device": ([vp, vp, sz], ctypes.c_int),
         "sf_device_synchronize": ([], ctypes.c_int),
         "sf_get_tuning": ([ctypes.POINTER(Tuning)], None),
         "sf_set_tuning": ([ctypes.POINTER(Tuning)], ctypes.c_int),
@@ -247,6 +248,14 @@ class DeviceArray:
         if a.nbytes > self.nbytes:
             raise ValueError("array larger than the allocation")
         _check(lib().sf_copy_to_device(self.ptr, _ptr(a), a.nbytes))
+
+    def copy_device(self, src, nbytes=None):
+        """Copies `nbytes` (all of `src` by default) from another DeviceArray,
+        device to device."""
+        n = src.nbytes if nbytes is None else int(nbytes)
+        if n > self.nbytes or n > src.nbytes:
+            raise ValueError("copy larger than an allocation")
+        _check(lib().sf_copy_on_device(self.ptr, src.ptr, n))
 
     def to_numpy(self, dtype, shape):
         out = np.empty(shape, dtype)

@@ -100,6 +100,7 @@ void *sf_device_alloc(size_t bytes);       /* NULL on failure */
 void sf_device_free(void *pointer);
 int sf_copy_to_device(void *dst, const void *src, size_t bytes);
 int sf_copy_to_host(void *dst, const void *src, size_t bytes);
+int sf_copy_on_device(void *dst, const void *src, size_t bytes);
 int sf_device_synchronize(void);
 
 /* ------------------------------------------------------------------------

@@ -120,6 +120,41 @@ def test_device_workspace_matches_find(device, synth_cases):
     ws.close()
 
 
+def test_fused_fast_path_wide_values(device, oracle_lib):
+    """Valid pixels anywhere below 2^18 stay on the fused kernel's fast path
+    (no exact re-run) and still match dext bit for bit: squares above 2^31,
+    values above 2^16 (their high bits travel in a separate ring), isolated
+    bright pixels on the first and last rows of 32-row segments, and the
+    EIGER-shaped frames, whose hot pixels under spots exceed 2^16."""
+    rng = np.random.default_rng(7)
+    ws = device.Workspace()
+    frames = []
+    for (h, w) in [(64, 96), (300, 465), (130, 700), (97, 233)]:
+        f = rng.poisson(1.0, (h, w)).astype(np.uint32)
+        s = rng.uniform(size=(h, w)) < 0.01
+        f[s] = rng.integers(40000, 1 << 18, s.sum())
+        for y in range(0, h, 32):
+            for x in rng.integers(0, w, 3):
+                f[y, x] = rng.choice([46341, 65535, 65536, (1 << 18) - 1])
+            if y + 31 < h:
+                f[y + 31, rng.integers(0, w)] = 70000
+        f[rng.uniform(size=(h, w)) < 0.01] = 0xFFFFFFFF
+        frames.append(("wide", f))
+    frames.append(("4m synth", device.synth_frame(device.synth_desc("4m"), 0)))
+    for name, f in frames:
+        assert f[f < 0xFFFFFFFE].max() < (1 << 18)
+        say("fast path", name, f.shape)
+        d = device.DeviceArray.from_numpy(f)
+        before = ws.fallbacks()
+        ws.threshold(d.ptr, 32, f.shape[0], f.shape[1], "fused")
+        rc, n = ws.sync()
+        got = ws.fetch(n)
+        _, want = oracle_lib.dext(f)
+        assert ws.fallbacks() == before, (name, "re-ran on the exact path")
+        assert rc == 0 and got.tobytes() == want.tobytes(), name
+    ws.close()
+
+
 def test_stage_times(device):
     device.profile_stages(True)
     try:
