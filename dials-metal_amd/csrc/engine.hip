@@ -81,13 +81,16 @@ void Engine::grow(DeviceBuffer &b, size_t bytes) {
 void Engine::ensure_slots(size_t n) {
   while (slots_.size() < n)
     slots_.push_back(std::make_unique<FrameSlot>());
-  grow(counters_, 4 * sizeof(uint32_t) * slots_.size());
-  if (host_counter_words_ < 4 * slots_.size()) {
+  // The slots already in use keep their counts: a frame's labelling reads
+  // the previous frame's.
+  if (counters_.ensure_keep(kCounters * sizeof(uint32_t) * slots_.size(), stream_))
+    epoch_++;
+  if (host_counter_words_ < kCounters * slots_.size()) {
     check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
     if (host_counters_)
       check(hipHostFree(host_counters_), "hipHostFree");
     host_counters_ = nullptr;
-    host_counter_words_ = 4 * slots_.size();
+    host_counter_words_ = kCounters * slots_.size();
     check(hipHostMalloc(reinterpret_cast<void **>(&host_counters_),
                         host_counter_words_ * sizeof(uint32_t),
                         hipHostMallocDefault),
@@ -144,6 +147,7 @@ void Engine::shape_label(FrameSlot &s) {
   grow(s.roots, s.root_words * 4);
   grow(s.root_prefix, s.root_words * 4);
   grow(s.root_totals, scan_tmp_words(s.root_words) * 4);
+  grow(s.row_runs, (static_cast<size_t>(s.height) + 1) * 4);
   grow(s.comp_of, (cap + 1) * 4);
   grow(s.comps, (cap + 1) * sizeof(sf_component));
   s.link_capacity = static_cast<uint32_t>(cap + 1);
@@ -155,7 +159,8 @@ FrameDev Engine::dev(int id, int prev) {
   FrameDev f{};
   f.unordered = s.unordered.as<sf_signal_pixel>();
   f.ordered = s.ordered.as<sf_signal_pixel>();
-  f.counter = counters_.as<uint32_t>() + 4 * id;
+  f.runs = s.unordered.as<uint4>();
+  f.counter = counters_.as<uint32_t>() + kCounters * id;
   f.capacity = s.capacity;
   f.width = s.width;
   f.height = s.height;
@@ -166,6 +171,7 @@ FrameDev Engine::dev(int id, int prev) {
   f.roots = s.roots.as<uint32_t>();
   f.root_prefix = s.root_prefix.as<uint32_t>();
   f.comp_of = s.comp_of.as<uint32_t>();
+  f.row_runs = s.row_runs.as<uint32_t>();
   f.comps = s.comps.as<sf_component>();
   f.links = s.links.as<uint32_t>();
   f.link_capacity = s.link_capacity;
@@ -174,6 +180,8 @@ FrameDev Engine::dev(int id, int prev) {
     f.prev_mask = p.mask.as<uint32_t>();
     f.prev_prefix = p.prefix.as<uint32_t>();
     f.prev_comp_of = p.comp_of.as<uint32_t>();
+    f.prev_counter = counters_.as<uint32_t>() + kCounters * prev;
+    f.prev_capacity = p.capacity;
   }
   return f;
 }
@@ -188,7 +196,7 @@ ScanSeg Engine::root_seg(int id) {
   FrameSlot &s = slot(id);
   return ScanSeg{s.roots.as<uint32_t>(), s.root_prefix.as<uint32_t>(),
                  s.root_totals.as<uint32_t>(), s.root_words,
-                 counters_.as<uint32_t>() + 4 * id, s.capacity};
+                 counters_.as<uint32_t>() + kCounters * id, s.capacity};
 }
 
 FrameArgs Engine::args(int id, const void *frame) {
@@ -199,7 +207,7 @@ FrameArgs Engine::args(int id, const void *frame) {
   a.width = s.width;
   a.bits = s.bits;
   a.out = s.unordered.as<sf_signal_pixel>();
-  a.counter = counters_.as<uint32_t>() + 4 * id;
+  a.counter = counters_.as<uint32_t>() + kCounters * id;
   a.capacity = s.capacity;
   a.mask = s.mask.as<uint32_t>();
   a.wpr = s.wpr;
@@ -345,7 +353,7 @@ void Engine::threshold(const FrameIn *frames, const int *ids, size_t n,
 
 void Engine::read_counters() {
   check(hipMemcpyAsync(host_counters_, counters_.ptr,
-                       4 * sizeof(uint32_t) * slots_.size(),
+                       kCounters * sizeof(uint32_t) * slots_.size(),
                        hipMemcpyDeviceToHost, stream_),
         "reading the counters back");
   check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
@@ -355,7 +363,7 @@ void Engine::sync(const int *ids, size_t n, size_t *counts, int *status) {
   read_counters();
   for (size_t i = 0; i < n; i++) {
     FrameSlot &s = slot(ids[i]);
-    if (host_counters_[4 * ids[i] + 1] != 0 && s.kernel == SF_KERNEL_FUSED) {
+    if (host_counters_[kCounters * ids[i] + 1] != 0 && s.kernel == SF_KERNEL_FUSED) {
       // A valid pixel beyond the fused kernel's exact range: this frame is
       // thresholded again, in its own slot, by the direct kernels, whose sums
       // are in the reference's types.
@@ -370,7 +378,7 @@ void Engine::sync(const int *ids, size_t n, size_t *counts, int *status) {
     }
   }
   for (size_t i = 0; i < n; i++) {
-    const uint32_t c = host_counters_[4 * ids[i]];
+    const uint32_t c = host_counters_[kCounters * ids[i]];
     if (counts)
       counts[i] = c;
     if (status)
@@ -393,9 +401,11 @@ void Engine::load_signal(int id, const sf_signal_pixel *list, size_t n,
   FrameSlot &s = slot(id);
   shape(s, 32, height, width);
   if (n > s.capacity) {
-    // Labelling a list never needs the threshold's bound; size for it.
+    // Labelling a list never needs the threshold's bound; size for it (the
+    // unordered list's memory holds the labelling's runs).
     s.capacity = static_cast<uint32_t>(n);
     grow(s.ordered, n * sizeof(sf_signal_pixel));
+    grow(s.unordered, n * sizeof(sf_signal_pixel));
   }
   s.kernel = SF_KERNEL_DIRECT;
   const size_t b_frames = align16(sizeof(FrameDev));
@@ -412,7 +422,7 @@ void Engine::load_signal(int id, const sf_signal_pixel *list, size_t n,
   staged();
   const FrameDev *fd_d = reinterpret_cast<const FrameDev *>(d);
   launch_clear(fd_d, 1, true, 0, stream_);
-  check(hipMemcpyAsync(counters_.as<uint32_t>() + 4 * id, d + b_frames + b_segs,
+  check(hipMemcpyAsync(counters_.as<uint32_t>() + kCounters * id, d + b_frames + b_segs,
                        sizeof(uint32_t), hipMemcpyDeviceToDevice, stream_),
         "setting the signal count");
   if (n)
@@ -460,7 +470,7 @@ void Engine::label(const int *ids, const int *prev, size_t n) {
 
 void Engine::label_counts(int id, size_t *n, size_t *ncomp, size_t *nlinks) {
   read_counters();
-  const uint32_t *c = host_counters_ + 4 * id;
+  const uint32_t *c = host_counters_ + kCounters * id;
   if (c[3] > slot(id).link_capacity)
     throw device_error("more frame links than the link list holds");
   if (n)

@@ -5,17 +5,29 @@
 // src/spots/dials_spots.cc:57-145): pixels are unioned with their left
 // neighbour (same row) and the one above, toward the lower position, and a
 // component's rows are emitted in order of its first pixel.  Here the same
-// forest is built in the rank space of each frame's ordered signal list:
-//   * the left neighbour of rank i, when set, is rank i - 1; the one above is
-//     found from the strong-pixel bitmask and its word prefix;
-//   * a lock-free union-find hooks the larger root under the smaller, so the
-//     root of every component is its lowest rank, i.e. its first pixel;
+// components are built from row runs -- maximal horizontal segments of set
+// bits in the strong-pixel bitmask -- in the rank space of each frame's
+// ordered signal list:
+//   * one wave per bitmask row counts the row's runs (a run starts at a set
+//     bit whose left neighbour is clear), a scan over the rows places each
+//     row's runs in the frame's run list, and a second pass lists each run
+//     {first rank, x0, y, length} and points its pixels at its first rank
+//     (no atomic: a returning atomic per row on one counter serialises the
+//     rows behind it);
+//   * each run is unioned with the runs above it that it touches, found as
+//     segments of the bitmask row above over its columns; a lock-free
+//     union-find hooks the larger root under the smaller, so the root of a
+//     component is its lowest rank, i.e. its first pixel;
 //   * roots are numbered by a popcount scan of a root bitmask, which numbers
 //     components in order of first appearance -- the reference's order;
-//   * every pixel adds its exact integer moments to its component with
-//     atomics (128-bit where a product can exceed 64 bits).
-// Links to the previous frame (same index set in both frames) are listed for
-// the host's three-dimensional stitch (dials_spots.cc:128-137).
+//   * each run sums its pixels' exact integer moments in registers and adds
+//     them to its component with one set of atomics (128-bit where a product
+//     can exceed 64 bits), and writes its pixels' component.
+// Links to the previous frame (pixels at the same position in both frames)
+// are listed per run, as the components of the previous frame's bitmask
+// segments under the run, for the host's three-dimensional stitch
+// (dials_spots.cc:128-137).  No step divides a pixel index by the width: rows
+// and columns come from the bitmask's layout.
 //
 // Every kernel works on frame blockIdx.y of a descriptor array, so a batch
 // is labelled in the same few launches as one frame.
@@ -24,19 +36,28 @@
 
 namespace {
 
+using sfd::gmem;
 using sfi::FrameDev;
 
-__device__ __forceinline__ uint32_t load_relaxed(const uint32_t *p) {
+constexpr int kLanes = 64;
+
+// A run record {first rank, x0, y, length} as the kernels move it.
+typedef uint32_t run4 __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ SF_GLOBAL run4 *run_list(const FrameDev &f) {
+  return reinterpret_cast<SF_GLOBAL run4 *>(gmem(f.runs));
+}
+
+__device__ __forceinline__ uint32_t load_relaxed(SF_GLOBAL uint32_t *p) {
   return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
-__device__ __forceinline__ void store_relaxed(uint32_t *p, uint32_t v) {
+__device__ __forceinline__ void store_relaxed(SF_GLOBAL uint32_t *p, uint32_t v) {
   __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 // The root of i, shortening the path as it goes.  Only non-roots are written,
 // and only with one of their ancestors, so a concurrent hook of a root is
 // never undone.
-__device__ uint32_t find(uint32_t *parent, uint32_t i) {
+__device__ uint32_t find(SF_GLOBAL uint32_t *parent, uint32_t i) {
   uint32_t cur = load_relaxed(&parent[i]);
   if (cur == i)
     return i;
@@ -51,7 +72,7 @@ __device__ uint32_t find(uint32_t *parent, uint32_t i) {
   }
 }
 
-__device__ void unite(uint32_t *parent, uint32_t a, uint32_t b) {
+__device__ void unite(SF_GLOBAL uint32_t *parent, uint32_t a, uint32_t b) {
   uint32_t ra = find(parent, a), rb = find(parent, b);
   while (ra != rb) {
     if (ra < rb) {
@@ -59,24 +80,14 @@ __device__ void unite(uint32_t *parent, uint32_t a, uint32_t b) {
       ra = rb;
       rb = t;
     }
-    const uint32_t old = atomicCAS(&parent[ra], ra, rb);
-    if (old == ra)
+    uint32_t expected = ra;
+    if (__hip_atomic_compare_exchange_strong(&parent[ra], &expected, rb,
+                                             __ATOMIC_RELAXED, __ATOMIC_RELAXED,
+                                             __HIP_MEMORY_SCOPE_AGENT))
       return;
-    ra = find(parent, old);
+    ra = find(parent, expected);
     rb = find(parent, rb);
   }
-}
-
-__device__ __forceinline__ bool bit_set(const uint32_t *mask, uint32_t wpr,
-                                        uint32_t x, uint32_t y) {
-  return (mask[static_cast<size_t>(y) * wpr + x / 32] >> (x & 31u)) & 1u;
-}
-
-__device__ __forceinline__ uint32_t rank_of(const uint32_t *mask,
-                                            const uint32_t *prefix, uint32_t wpr,
-                                            uint32_t x, uint32_t y) {
-  const size_t w = static_cast<size_t>(y) * wpr + x / 32;
-  return prefix[w] + static_cast<uint32_t>(__popc(mask[w] & ((1u << (x & 31u)) - 1u)));
 }
 
 // An overfull frame is refused, not labelled.
@@ -85,31 +96,181 @@ __device__ __forceinline__ uint32_t live_count(const FrameDev &f) {
   return n > f.capacity ? 0u : n;
 }
 
-#define FOR_EACH_PIXEL(i, n)                                                   \
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < (n);            \
-       i += gridDim.x * blockDim.x)
+// Bits of bitmask word j that fall in columns [x0, x1).
+__device__ __forceinline__ uint32_t columns(uint32_t j, uint32_t x0, uint32_t x1) {
+  const uint32_t base = 32u * j;
+  const uint32_t lo = x0 > base ? x0 - base : 0u;
+  const uint32_t hi = x1 < base + 32u ? x1 - base : 32u;
+  const uint32_t upto = hi == 32u ? 0xFFFFFFFFu : (1u << hi) - 1u;
+  return upto & ~((1u << lo) - 1u);
+}
 
-// parent[i] = i - 1 when the left neighbour is set, else i.
-__global__ __launch_bounds__(256) void link_left(const FrameDev *__restrict__ frames) {
+// Wave-wide exclusive prefix of v and the wave's total.
+__device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t lane,
+                                                uint32_t *total) {
+  uint32_t incl = v;
+#pragma unroll
+  for (int d = 1; d < kLanes; d <<= 1) {
+    const uint32_t o = __shfl_up(incl, d);
+    if (lane >= static_cast<uint32_t>(d))
+      incl += o;
+  }
+  *total = __shfl(incl, kLanes - 1);
+  return incl - v;
+}
+
+// Runs starting in this lane's words of bitmask row `row` (words lane,
+// lane + 64, ...): a run starts at a set bit whose left neighbour in the row
+// is clear.
+__device__ __forceinline__ uint32_t lane_runs(const SF_GLOBAL uint32_t *row,
+                                              uint32_t wpr, uint32_t lane) {
+  uint32_t count = 0;
+  for (uint32_t j = lane; j < wpr; j += kLanes) {
+    const uint32_t m = row[j];
+    if (m == 0)
+      continue;
+    const uint32_t carry = j > 0 ? row[j - 1] >> 31 : 0u;
+    count += static_cast<uint32_t>(__popc(m & ~((m << 1) | carry)));
+  }
+  return count;
+}
+
+// Runs per bitmask row of every frame, one wave per row.
+__global__ __launch_bounds__(256) void count_runs(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
-  const uint32_t n = live_count(f);
-  FOR_EACH_PIXEL(i, n) {
-    const uint32_t idx = f.ordered[i].index;
-    const bool left =
-        i > 0 && idx % f.width != 0 && f.ordered[i - 1].index == idx - 1;
-    f.parent[i] = left ? i - 1 : i;
+  if (live_count(f) == 0)
+    return;
+  const uint32_t lane = threadIdx.x % kLanes;
+  const uint32_t per_block = blockDim.x / kLanes;
+  for (uint32_t y = blockIdx.x * per_block + threadIdx.x / kLanes; y < f.height;
+       y += gridDim.x * per_block) {
+    const uint32_t count =
+        lane_runs(gmem(f.mask) + static_cast<size_t>(y) * f.wpr, f.wpr, lane);
+    uint32_t total;
+    (void)wave_prefix(count, lane, &total);
+    if (lane == 0)
+      gmem(f.row_runs)[y] = total;
   }
 }
 
-// Union with the pixel above.
-__global__ __launch_bounds__(256) void link_up(const FrameDev *__restrict__ frames) {
+// Each row's first run (an exclusive scan of the runs per row, in place) and
+// the frame's run count; one block of 1024 threads per frame, each thread a
+// contiguous share of the rows.
+__global__ __launch_bounds__(1024) void scan_runs(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.x];
+  __shared__ uint32_t wave_sums[1024 / kLanes];
+  if (live_count(f) == 0) {
+    if (threadIdx.x == 0)
+      f.counter[4] = 0;
+    return;
+  }
+  SF_GLOBAL uint32_t *rows = gmem(f.row_runs);
+  const uint32_t per = (f.height + blockDim.x - 1) / blockDim.x;
+  const uint32_t y0 = threadIdx.x * per;
+  const uint32_t y1 = min(f.height, y0 + per);
+  uint32_t mine = 0;
+  for (uint32_t y = y0; y < y1; y++)
+    mine += rows[y];
+  const uint32_t lane = threadIdx.x % kLanes, wave = threadIdx.x / kLanes;
+  uint32_t total;
+  uint32_t before = wave_prefix(mine, lane, &total);
+  if (lane == 0)
+    wave_sums[wave] = total;
+  __syncthreads();
+  uint32_t all = 0;
+  for (uint32_t w = 0; w < blockDim.x / kLanes; w++) {
+    if (w < wave)
+      before += wave_sums[w];
+    all += wave_sums[w];
+  }
+  for (uint32_t y = y0; y < y1; y++) {
+    const uint32_t c = rows[y];
+    rows[y] = before;
+    before += c;
+  }
+  if (threadIdx.x == 0)
+    f.counter[4] = all;
+}
+
+// The runs of every frame, one wave per bitmask row, each row's at its place
+// in the frame's run list; every pixel of a run points at the run's first
+// rank (the union-find's initial forest).
+__global__ __launch_bounds__(256) void emit_runs(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
-  const uint32_t n = live_count(f);
-  FOR_EACH_PIXEL(i, n) {
-    const uint32_t idx = f.ordered[i].index;
-    const uint32_t y = idx / f.width, x = idx - y * f.width;
-    if (y > 0 && bit_set(f.mask, f.wpr, x, y - 1))
-      unite(f.parent, i, rank_of(f.mask, f.prefix, f.wpr, x, y - 1));
+  if (live_count(f) == 0)
+    return;
+  const uint32_t lane = threadIdx.x % kLanes;
+  const uint32_t per_block = blockDim.x / kLanes;
+  SF_GLOBAL uint32_t *parent = gmem(f.parent);
+  SF_GLOBAL run4 *runs = run_list(f);
+  for (uint32_t y = blockIdx.x * per_block + threadIdx.x / kLanes; y < f.height;
+       y += gridDim.x * per_block) {
+    const SF_GLOBAL uint32_t *row = gmem(f.mask) + static_cast<size_t>(y) * f.wpr;
+    const SF_GLOBAL uint32_t *pre = gmem(f.prefix) + static_cast<size_t>(y) * f.wpr;
+    uint32_t total;
+    uint32_t k = wave_prefix(lane_runs(row, f.wpr, lane), lane, &total);
+    if (total == 0)
+      continue;
+    k += gmem(f.row_runs)[y];
+    for (uint32_t j = lane; j < f.wpr; j += kLanes) {
+      const uint32_t m = row[j];
+      if (m == 0)
+        continue;
+      const uint32_t carry = j > 0 ? row[j - 1] >> 31 : 0u;
+      uint32_t starts = m & ~((m << 1) | carry);
+      while (starts) {
+        const uint32_t b = static_cast<uint32_t>(__ffs(starts)) - 1u;
+        starts &= starts - 1u;
+        const uint32_t r0 = pre[j] + static_cast<uint32_t>(__popc(m & ((1u << b) - 1u)));
+        // The run's length: set bits from b on, into the row's next words.
+        uint32_t len = 0, jj = j, w = m >> b, avail = 32u - b;
+        for (;;) {
+          const uint32_t ones =
+              ~w == 0u ? 32u : static_cast<uint32_t>(__ffs(~w)) - 1u;
+          const uint32_t t = ones < avail ? ones : avail;
+          len += t;
+          if (t < avail || ++jj >= f.wpr)
+            break;
+          w = row[jj];
+          avail = 32u;
+        }
+        runs[k++] = run4{r0, 32u * j + b, y, len};
+        for (uint32_t r = r0; r < r0 + len; r++)
+          parent[r] = r0;
+      }
+    }
+  }
+}
+
+// Unites each run with the runs above it that share a column with it: one
+// union per segment of the row above within the run's columns.
+__global__ __launch_bounds__(256) void unite_runs(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t nr = f.counter[4];
+  SF_GLOBAL uint32_t *parent = gmem(f.parent);
+  const SF_GLOBAL run4 *runs = run_list(f);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
+       k += gridDim.x * blockDim.x) {
+    const run4 r = runs[k]; // r0, x0, y, length
+    if (r.z == 0)
+      continue;
+    const uint32_t x1 = r.y + r.w;
+    const size_t above = static_cast<size_t>(r.z - 1) * f.wpr;
+    const SF_GLOBAL uint32_t *row = gmem(f.mask) + above;
+    const SF_GLOBAL uint32_t *pre = gmem(f.prefix) + above;
+    uint32_t carry = 0;
+    for (uint32_t j = r.y >> 5; j <= (x1 - 1u) >> 5; j++) {
+      const uint32_t full = row[j];
+      const uint32_t a = full & columns(j, r.y, x1);
+      uint32_t segs = a & ~((a << 1) | carry);
+      carry = a >> 31;
+      while (segs) {
+        const uint32_t b = static_cast<uint32_t>(__ffs(segs)) - 1u;
+        segs &= segs - 1u;
+        unite(parent, r.x,
+              pre[j] + static_cast<uint32_t>(__popc(full & ((1u << b) - 1u))));
+      }
+    }
   }
 }
 
@@ -120,51 +281,56 @@ __global__ __launch_bounds__(256) void flatten(const FrameDev *__restrict__ fram
   const FrameDev f = frames[blockIdx.y];
   const uint32_t n = live_count(f);
   const uint32_t span = (n + 63u) & ~63u;
-  FOR_EACH_PIXEL(i, span) {
+  SF_GLOBAL uint32_t *parent = gmem(f.parent);
+  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < span;
+       i += gridDim.x * blockDim.x) {
     bool root = false;
     if (i < n) {
-      uint32_t r = load_relaxed(&f.parent[i]);
+      uint32_t r = load_relaxed(&parent[i]);
       for (;;) {
-        const uint32_t up = load_relaxed(&f.parent[r]);
+        const uint32_t up = load_relaxed(&parent[r]);
         if (up == r)
           break;
         r = up;
       }
-      store_relaxed(&f.parent[i], r);
+      store_relaxed(&parent[i], r);
       root = r == i;
     }
     const uint64_t b = __ballot(root);
     const uint32_t lane = sfd::lane_id();
     if (lane == 0)
-      f.roots[i / 32] = static_cast<uint32_t>(b);
+      gmem(f.roots)[i / 32] = static_cast<uint32_t>(b);
     else if (lane == 32)
-      f.roots[i / 32] = static_cast<uint32_t>(b >> 32);
+      gmem(f.roots)[i / 32] = static_cast<uint32_t>(b >> 32);
   }
 }
 
+// Roots strictly before rank r: the number of the component rooted at r.
 __device__ __forceinline__ uint32_t component_of(const FrameDev &f, uint32_t r) {
-  return f.root_prefix[r / 32] +
-         static_cast<uint32_t>(__popc(f.roots[r / 32] & ((1u << (r & 31u)) - 1u)));
+  return gmem(f.root_prefix)[r / 32] +
+         static_cast<uint32_t>(__popc(gmem(f.roots)[r / 32] & ((1u << (r & 31u)) - 1u)));
 }
 
-// Component of every pixel; each root opens its component's record, and the
-// last pixel, which knows how many roots precede it, writes the count.
-__global__ __launch_bounds__(256) void number_components(
+// Each root run opens its component's record; the run holding the last rank
+// writes the component count.
+__global__ __launch_bounds__(256) void open_components(
     const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
+  const uint32_t nr = f.counter[4];
   const uint32_t n = live_count(f);
-  FOR_EACH_PIXEL(i, n) {
-    const uint32_t r = f.parent[i];
-    const uint32_t c = component_of(f, r);
-    f.comp_of[i] = c;
-    if (r == i) {
+  const SF_GLOBAL uint32_t *parent = gmem(f.parent);
+  const SF_GLOBAL run4 *runs = run_list(f);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
+       k += gridDim.x * blockDim.x) {
+    const run4 r = runs[k];
+    if (parent[r.x] == r.x) {
       sf_component z{};
-      z.first = f.ordered[i].index;
+      z.first = r.z * f.width + r.y;
       z.x0 = z.y0 = 0xFFFFFFFFu;
-      f.comps[c] = z;
+      f.comps[component_of(f, r.x)] = z;
     }
-    if (i == n - 1)
-      f.counter[2] = component_of(f, i) + (r == i ? 1u : 0u);
+    if (r.x + r.w == n)
+      f.counter[2] = component_of(f, n - 1) + (parent[n - 1] == n - 1 ? 1u : 0u);
   }
 }
 
@@ -186,97 +352,139 @@ __device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t l,
   lo = s;
 }
 
-// Exact moments into the components, one row-run at a time: the pixel that
-// starts a run (no set pixel to its left) sums the run in registers, then
-// adds the run to its component with one set of atomics.  A run lies in one
-// row and one component, and runs are short, so the component records see a
-// few atomics per row of a spot rather than one set per pixel.
+// Exact moments into the components, one run at a time: the run's pixels are
+// consecutive ranks of the ordered list, summed in registers, then added to
+// the component with one set of atomics; every pixel's component is written.
 __global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
-  const uint32_t n = live_count(f);
-  FOR_EACH_PIXEL(i, n) {
-    const uint32_t first = f.ordered[i].index;
-    const uint32_t y = first / f.width, x0 = first - y * f.width;
-    if (x0 > 0 && i > 0 && f.ordered[i - 1].index == first - 1)
-      continue; // not the start of a run
-    const uint64_t ay = 2ull * y + 1ull;
-    uint32_t count = 0, x = x0;
+  const uint32_t nr = f.counter[4];
+  const SF_GLOBAL uint32_t *parent = gmem(f.parent);
+  const SF_GLOBAL run4 *runs = run_list(f);
+  const SF_GLOBAL sf_signal_pixel *ordered = gmem(f.ordered);
+  SF_GLOBAL uint32_t *comp_of = gmem(f.comp_of);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
+       k += gridDim.x * blockDim.x) {
+    const run4 r = runs[k]; // r0, x0, y, length
+    const uint32_t c = component_of(f, parent[r.x]);
+    const uint32_t first = r.z * f.width + r.y;
+    const uint64_t ay = 2ull * r.z + 1ull;
     uint64_t peak = 0, sum = 0;
     uint64_t sq_lo = 0, sq_hi = 0, sx_lo = 0, sx_hi = 0, xx_lo = 0, xx_hi = 0;
-    uint32_t j = i;
-    for (;;) {
-      const uint64_t v = f.ordered[j].value;
-      const uint64_t ax = 2ull * x + 1ull;
-      const uint64_t key = (v << 32) | (0xFFFFFFFFu - (first + count));
+    for (uint32_t j = 0; j < r.w; j++) {
+      const uint64_t v = ordered[r.x + j].value;
+      comp_of[r.x + j] = c;
+      const uint64_t ax = 2ull * (r.y + j) + 1ull;
+      const uint64_t key = (v << 32) | (0xFFFFFFFFu - (first + j));
       peak = key > peak ? key : peak;
       sum += v;
       acc128(sq_lo, sq_hi, v * v, 0);
       acc128(sx_lo, sx_hi, v * ax, 0);
       const uint64_t xx = ax * ax;
       acc128(xx_lo, xx_hi, v * xx, __umul64hi(v, xx));
-      count++;
-      j++;
-      x++;
-      if (j >= n || x >= f.width || f.ordered[j].index != first + count)
-        break;
     }
-    sf_component *c = &f.comps[f.comp_of[i]];
-    atomicAdd(&c->n, count);
-    atomicMin(&c->x0, x0);
-    atomicMax(&c->x1, x0 + count);
-    atomicMin(&c->y0, y);
-    atomicMax(&c->y1, y + 1);
-    atomicMax(reinterpret_cast<unsigned long long *>(&c->peak),
+    sf_component *cp = &f.comps[c];
+    atomicAdd(&cp->n, r.w);
+    atomicMin(&cp->x0, r.y);
+    atomicMax(&cp->x1, r.y + r.w);
+    atomicMin(&cp->y0, r.z);
+    atomicMax(&cp->y1, r.z + 1);
+    atomicMax(reinterpret_cast<unsigned long long *>(&cp->peak),
               static_cast<unsigned long long>(peak));
-    atomicAdd(reinterpret_cast<unsigned long long *>(&c->sum),
+    atomicAdd(reinterpret_cast<unsigned long long *>(&cp->sum),
               static_cast<unsigned long long>(sum));
-    add128(c->sum_sq, sq_lo, sq_hi);
-    add128(c->sum_x, sx_lo, sx_hi);
+    add128(cp->sum_sq, sq_lo, sq_hi);
+    add128(cp->sum_x, sx_lo, sx_hi);
     // sum of value * ay and value * ay^2 over the run: ay is the run's
     // constant, so these are products of the run's sum.
-    add128(c->sum_y, sum * ay, __umul64hi(sum, ay));
+    add128(cp->sum_y, sum * ay, __umul64hi(sum, ay));
     const uint64_t yy = ay * ay;
-    add128(c->sum_yy, sum * yy, __umul64hi(sum, yy));
-    add128(c->sum_xx, xx_lo, xx_hi);
+    add128(cp->sum_yy, sum * yy, __umul64hi(sum, yy));
+    add128(cp->sum_xx, xx_lo, xx_hi);
   }
 }
 
-// Pairs (component here, component on the previous frame) for every pixel
-// set in both frames; a pixel repeating its left neighbour's pair is skipped.
+// Pairs (component here, component on the previous frame) for the pixels set
+// in both frames: per run, the previous frame's components along its columns,
+// each pair once per run (a component repeated along the run is listed once).
+// One atomic per wave on the frame's link count.
 __global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   if (f.prev_mask == nullptr)
     return;
-  const uint32_t n = live_count(f);
-  FOR_EACH_PIXEL(i, n) {
-    const uint32_t idx = f.ordered[i].index;
-    const uint32_t y = idx / f.width, x = idx - y * f.width;
-    bool emit = false;
-    uint32_t here = 0, there = 0;
-    if (bit_set(f.prev_mask, f.wpr, x, y)) {
-      here = f.comp_of[i];
-      there = f.prev_comp_of[rank_of(f.prev_mask, f.prev_prefix, f.wpr, x, y)];
-      emit = !(x > 0 && i > 0 && f.ordered[i - 1].index == idx - 1 &&
-               f.comp_of[i - 1] == here && bit_set(f.prev_mask, f.wpr, x - 1, y) &&
-               f.prev_comp_of[rank_of(f.prev_mask, f.prev_prefix, f.wpr, x - 1,
-                                      y)] == there);
+  // A previous frame refused as overfull has no component list to link to.
+  const uint32_t prev_n = f.prev_counter[0];
+  if (prev_n > f.prev_capacity)
+    return;
+  const uint32_t nr = f.counter[4];
+  const uint32_t lane = threadIdx.x % kLanes;
+  const SF_GLOBAL run4 *runs = run_list(f);
+  const SF_GLOBAL uint32_t *comp_of = gmem(f.comp_of);
+  const SF_GLOBAL uint32_t *prev_comp = gmem(f.prev_comp_of);
+  // Wave-uniform steps over the runs, so that whole waves meet at the scan.
+  for (uint32_t step = blockIdx.x * blockDim.x + (threadIdx.x & ~(kLanes - 1u));
+       step < nr; step += gridDim.x * blockDim.x) {
+    const uint32_t k = step + lane;
+    const bool active = k < nr;
+    run4 r = {0u, 0u, 0u, 0u};
+    if (active)
+      r = runs[k];
+    const size_t off = static_cast<size_t>(r.z) * f.wpr;
+    const SF_GLOBAL uint32_t *row = gmem(f.prev_mask) + off;
+    const SF_GLOBAL uint32_t *pre = gmem(f.prev_prefix) + off;
+    const uint32_t x1 = r.y + r.w;
+    // Two passes over the run's columns: count the distinct consecutive
+    // components, then write them at the wave's place in the list.
+    uint32_t here = 0, count = 0;
+    if (active) {
+      here = comp_of[r.x];
+      uint32_t carry = 0, last = 0xFFFFFFFFu;
+      for (uint32_t j = r.y >> 5; j <= (x1 - 1u) >> 5; j++) {
+        const uint32_t full = row[j];
+        const uint32_t a = full & columns(j, r.y, x1);
+        uint32_t segs = a & ~((a << 1) | carry);
+        carry = a >> 31;
+        while (segs) {
+          const uint32_t b = static_cast<uint32_t>(__ffs(segs)) - 1u;
+          segs &= segs - 1u;
+          const uint32_t q =
+              pre[j] + static_cast<uint32_t>(__popc(full & ((1u << b) - 1u)));
+          const uint32_t there = q < prev_n ? prev_comp[q] : last;
+          count += there != last ? 1u : 0u;
+          last = there;
+        }
+      }
     }
-    // One atomic on the frame's link count per wave, not per link.
-    const uint64_t b = __ballot(emit);
-    if (b == 0)
+    uint32_t total;
+    uint32_t at = wave_prefix(count, lane, &total);
+    if (total == 0)
       continue;
-    const uint32_t lane = sfd::lane_id();
-    const int leader = __ffsll(static_cast<unsigned long long>(b)) - 1;
     uint32_t base = 0;
-    if (static_cast<int>(lane) == leader)
-      base = atomicAdd(&f.counter[3], static_cast<uint32_t>(__popcll(b)));
-    base = __shfl(base, leader);
-    if (emit) {
-      const uint32_t at =
-          base + static_cast<uint32_t>(__popcll(b & ((1ull << lane) - 1ull)));
-      if (at < f.link_capacity) {
-        f.links[2 * at] = here;
-        f.links[2 * at + 1] = there;
+    if (lane == 0)
+      base = __hip_atomic_fetch_add(gmem(f.counter) + 3, total, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    at += __shfl(base, 0);
+    if (count) {
+      uint32_t carry = 0, last = 0xFFFFFFFFu;
+      for (uint32_t j = r.y >> 5; j <= (x1 - 1u) >> 5; j++) {
+        const uint32_t full = row[j];
+        const uint32_t a = full & columns(j, r.y, x1);
+        uint32_t segs = a & ~((a << 1) | carry);
+        carry = a >> 31;
+        while (segs) {
+          const uint32_t b = static_cast<uint32_t>(__ffs(segs)) - 1u;
+          segs &= segs - 1u;
+          const uint32_t q =
+              pre[j] + static_cast<uint32_t>(__popc(full & ((1u << b) - 1u)));
+          const uint32_t there = q < prev_n ? prev_comp[q] : last;
+          if (there != last) {
+            if (at < f.link_capacity) {
+              gmem(f.links)[2 * at] = here;
+              gmem(f.links)[2 * at + 1] = there;
+            }
+            at++;
+          }
+          last = there;
+        }
       }
     }
   }
@@ -287,18 +495,20 @@ __global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ 
 namespace sfi {
 
 // frames_device: the frames' descriptors; root_segs_device: their root-bit
-// scans; both in device memory, nframes long.  counter[2] and counter[3] of
-// every frame are cleared by the caller.
+// scans; both in device memory, nframes long.  counter[2], counter[3] and
+// counter[4] of every frame are cleared by the caller.
 void launch_label(const FrameDev *frames_device, const ScanSeg *root_segs_device,
                   uint32_t nframes, size_t max_root_words, hipStream_t stream) {
   if (nframes == 0)
     return;
   const dim3 grid(256, nframes);
-  link_left<<<grid, 256, 0, stream>>>(frames_device);
-  link_up<<<grid, 256, 0, stream>>>(frames_device);
+  count_runs<<<grid, 256, 0, stream>>>(frames_device);
+  scan_runs<<<nframes, 1024, 0, stream>>>(frames_device);
+  emit_runs<<<grid, 256, 0, stream>>>(frames_device);
+  unite_runs<<<grid, 256, 0, stream>>>(frames_device);
   flatten<<<grid, 256, 0, stream>>>(frames_device);
   launch_popcount_scan(root_segs_device, nframes, max_root_words, stream);
-  number_components<<<grid, 256, 0, stream>>>(frames_device);
+  open_components<<<grid, 256, 0, stream>>>(frames_device);
   accumulate<<<grid, 256, 0, stream>>>(frames_device);
   link_frames<<<grid, 256, 0, stream>>>(frames_device);
   check(hipGetLastError(), "launching the labelling");

@@ -162,7 +162,7 @@ __global__ __launch_bounds__(256) void clear_frames(
     *queue = 0;
   const FrameDev f = frames[blockIdx.y];
   if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) >= first &&
-      threadIdx.x < 4)
+      threadIdx.x < sfi::kCounters)
     f.counter[threadIdx.x] = 0;
   if (!masks)
     return;

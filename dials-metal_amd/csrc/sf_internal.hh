@@ -57,6 +57,23 @@ struct DeviceBuffer {
     bytes = want;
     return true;
   }
+  // Grows keeping the contents (the stream's work on the old buffer is
+  // waited for); true when the buffer moved.
+  bool ensure_keep(size_t want, hipStream_t stream) {
+    if (want <= bytes)
+      return false;
+    void *next = nullptr;
+    check(hipMalloc(&next, want), "hipMalloc");
+    if (ptr) {
+      check(hipMemcpyAsync(next, ptr, bytes, hipMemcpyDeviceToDevice, stream),
+            "keeping a buffer's contents");
+      check(hipStreamSynchronize(stream), "waiting before a buffer grows");
+      check(hipFree(ptr), "hipFree");
+    }
+    ptr = next;
+    bytes = want;
+    return true;
+  }
   template <typename P> P *as() const { return static_cast<P *>(ptr); }
 };
 
@@ -143,24 +160,35 @@ size_t scan_tmp_words(size_t words);
 void launch_popcount_scan(const ScanSeg *segs_device, uint32_t nframes,
                           size_t max_words, hipStream_t stream);
 
+// Words of counters per frame: [0] signal pixels [1] the fused kernel's
+// overflow flag [2] components [3] frame links [4] row runs.
+constexpr int kCounters = 8;
+
 // One frame's device state after its threshold, for ordering and labelling.
 struct FrameDev {
   sf_signal_pixel *unordered; // the threshold's emit
   sf_signal_pixel *ordered;   // ascending by index
-  uint32_t *counter;          // [0] pixels [1] overflow [2] components [3] links
+  // The labelling's row runs {first rank, x0, y, length}, in the unordered
+  // list's memory (free once the list is ordered; runs <= pixels).
+  uint4 *runs;
+  uint32_t *counter;          // kCounters words, as above
   uint32_t capacity;
   uint32_t width, height, wpr;
   uint32_t *mask, *prefix;    // strong-pixel bitmask and its word prefix
   // labelling
   uint32_t *parent, *roots, *root_prefix, *comp_of;
+  uint32_t *row_runs; // height words: runs per row, then each row's first run
   sf_component *comps;
-  // the previous frame of the same shape, for three-dimensional links
-  const uint32_t *prev_mask, *prev_prefix, *prev_comp_of;
+  // the previous frame of the same shape, for three-dimensional links; its
+  // counters say whether it was labelled (a refused, overfull frame has a
+  // bitmask but no list, so it is not linked to)
+  const uint32_t *prev_mask, *prev_prefix, *prev_comp_of, *prev_counter;
+  uint32_t prev_capacity;
   uint32_t *links;            // 2 * link_capacity words
   uint32_t link_capacity;
 };
 
-// Clears each frame's counters from `first` to the fourth and, with `masks`,
+// Clears each frame's counters from `first` to the last and, with `masks`,
 // its bitmask; zeroes `queue` when given (the fused kernel's item queue).
 void launch_clear(const FrameDev *frames_device, uint32_t nframes, bool masks,
                   int first, hipStream_t stream, uint32_t *queue = nullptr);
