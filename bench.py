@@ -55,6 +55,103 @@ def cpu_baseline(frames_host, threads):
                       f"pixels; threshold only, no grouping"}
 
 
+def serving(sf, seconds, seed):
+    """Six concurrent sweeps -- two each of EIGER2 4M, 9M and 16M uint32 --
+    emitting frames by Poisson arrival at 100-750 Hz each (rates drawn from a
+    seeded generator), replayed into the serving loop twice on the same
+    schedule: continuous batching (every frame that has arrived, across
+    sweeps, in one launch) and one frame per launch.  Threshold, ordering and
+    2D labelling linked to each sweep's previous frame, frames resident in
+    HBM (four distinct frames per sweep, cycled)."""
+    rng = np.random.default_rng(seed)
+    kinds = ["4m", "4m", "9m", "9m", "16m", "16m"]
+    rates = rng.uniform(100.0, 750.0, len(kinds))
+    descs, frames = [], []
+    for i, kind in enumerate(kinds):
+        d = sf.synth_desc(kind, seed=1000 + i,
+                          spots_per_frame=int(rng.integers(300, 2000)), persist=2)
+        descs.append(d)
+        frames.append([sf.DeviceArray.from_numpy(sf.synth_frame(d, z))
+                       for z in range(4)])
+    events = []
+    for i, r in enumerate(rates):
+        t = rng.exponential(1.0 / r)
+        while t < seconds:
+            events.append((t, i))
+            t += rng.exponential(1.0 / r)
+    events.sort()
+    taken = [0] * len(kinds)
+    ptrs = []
+    for _, i in events:
+        ptrs.append(frames[i][taken[i] % 4].ptr)
+        taken[i] += 1
+    arrivals = [t for t, _ in events]
+    out = {"sweeps": kinds, "rates_hz": [round(float(r), 1) for r in rates],
+           "seconds": seconds, "frames": len(events),
+           "target_p99_ms": 2.0}
+    for name, max_batch in (("batched", 64), ("plain_per_frame", 1)):
+        srv = sf.Server(max_batch=max_batch, ring=8, label=True)
+        ids = [srv.open_sweep(d.height, d.width, 32) for d in descs]
+        res = srv.replay([ids[i] for _, i in events], ptrs, arrivals)
+        bad = [r for r in res if r["status"] != 0]
+        if bad:
+            raise RuntimeError(f"serving: {len(bad)} frames refused")
+        st = srv.stats()
+        srv.close()
+        out[name] = {k: round(st[k], 4) for k in
+                     ("p50_ms", "p99_ms", "max_ms", "mean_ms", "frames_per_s",
+                      "mean_batch")}
+        log(f"serving {name}: {out[name]}")
+    return out
+
+
+def uneven_batch(sf, reps, graph):
+    """One launch over an interleaved stream of 4M, 9M and 16M uint32 frames
+    and 512 x 512 uint16 ROIs in the ratio 3:2:4:8, backgrounds with Poisson
+    means 0.2-5 and 100-5000 spots per frame: threshold, ordering and 2D
+    labelling; frames/s and the fused kernel's achieved HBM bandwidth over
+    the batch's own bytes."""
+    rng = np.random.default_rng(17)
+    kinds = ["4m"] * 3 + ["9m"] * 2 + ["16m"] * 4 + ["roi"] * 8
+    rng.shuffle(kinds)
+    dev, descs, byts = [], [], 0.0
+    for i, kind in enumerate(kinds):
+        bg = float(rng.uniform(0.2, 5.0))
+        if kind == "roi":
+            d = sf.synth_desc((512, 512), bits=16, seed=500 + i, background=bg,
+                              spots_per_frame=int(rng.integers(100, 400)),
+                              ring_radius=150.0)
+        else:
+            d = sf.synth_desc(kind, seed=500 + i, background=bg,
+                              spots_per_frame=int(rng.integers(100, 5000)))
+        f = sf.synth_frame(d, 0)
+        dev.append(sf.DeviceArray.from_numpy(f))
+        descs.append((dev[-1].ptr, d.height, d.width, d.bits))
+    batch = sf.Batch()
+    batch.threshold(descs, graph=graph)
+    batch.label(True)
+    counts, status = batch.sync()
+    for d, c in zip(descs, counts):
+        px = d[1] * d[2]
+        byts += px * d[3] / 8 + 16 * c + px / 8
+    sf.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        batch.threshold(descs, graph=graph)
+        batch.label(True)
+        batch.sync()
+    secs = (time.perf_counter() - t0) / reps
+    batch.set_timing(True)
+    batch.threshold(descs)
+    batch.sync()
+    kms = batch.kernel_ms()[0]
+    batch.set_timing(False)
+    return {"frames": len(kinds), "mix": "4M:9M:16M:ROI16 = 3:2:4:8",
+            "frames_per_s": len(kinds) / secs, "ms_per_batch": 1e3 * secs,
+            "fused_ms": kms, "fused_GBps": byts / (kms * 1e-3) / 1e9,
+            "refused": int(sum(1 for s in status if s))}
+
+
 def copy_calibration(sf, nbytes=1 << 30, copies=4):
     """Plain device-to-device copies: each reads and writes `nbytes`, so the
     HBM read and write counters of these dispatches give the factor from
@@ -84,6 +181,12 @@ def main():
     ap.add_argument("--no-label", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--autotune", type=int, default=1,
+                    help="measure the kernel shape on the benchmark's frames "
+                         "first (sf_autotune) and run with the best")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-seconds", type=float, default=2.0)
+    ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only device-to-device copies of 1 GiB, the known "
                          "traffic against which HBM counters are scaled")
@@ -109,6 +212,11 @@ def main():
     descs = [(d.ptr, h, w, 32) for d in dev]
     sf.synchronize()
     label = not args.no_label
+
+    tuned = None
+    if args.autotune:
+        tuned = sf.autotune(descs, label=not args.no_label)
+        log(f"autotune: {tuned}")
 
     batch = sf.Batch()
     counts = []
@@ -170,6 +278,7 @@ def main():
                            if label else ""),
             "frame": [h, w], "pixel": "uint32", "frames_per_step": len(dev),
             "graph": bool(args.graph), "tuning": sf.get_tuning(),
+            "autotune": tuned,
             "mean_signal_pixels": mean_signal},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
@@ -198,6 +307,12 @@ def main():
         ws.close()
         line["plain_per_frame"] = {"value": plain, "unit": "frames/s",
                                    "labelled": label}
+
+    if not args.no_uneven:
+        line["uneven_batch"] = uneven_batch(sf, reps=5, graph=bool(args.graph))
+        log(f"uneven batch: {line['uneven_batch']}")
+    if not args.no_serve:
+        line["serving"] = serving(sf, args.serve_seconds, seed=2025)
 
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(frames_host, min(16, os.cpu_count() or 1))

@@ -26,6 +26,7 @@ struct FrameSlot {
   bool thresholded = false, labelled = false;
   uint64_t serial = 0;  // which frame of the engine it holds
   uint64_t linked_serial = 0; // the frame its links were taken against, or 0
+  int linked_slot = -1;       // and that frame's slot
   const void *frame = nullptr;
   int kernel = 0;
   uint32_t link_capacity = 0;
@@ -71,7 +72,12 @@ public:
   // same shape, labelled).
   void label(const int *ids, const int *prev, size_t n);
   // Waits; pixel, component and link counts of one slot.
-  void label_counts(int id, size_t *n, size_t *ncomp, size_t *nlinks);
+  // `fresh` false reuses the counters the last sync() read (after a
+  // labelling enqueued before it).
+  void label_counts(int id, size_t *n, size_t *ncomp, size_t *nlinks,
+                    bool fresh = true);
+  // Device bytes a slot of this shape holds once labelled.
+  static size_t slot_bytes(uint32_t height, uint32_t width, int bits);
   void fetch_labels(int id, uint32_t *comp_of, size_t n, sf_component *comps,
                     size_t ncomp, uint32_t *links, size_t nlinks);
 

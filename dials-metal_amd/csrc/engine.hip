@@ -140,6 +140,17 @@ void Engine::shape(FrameSlot &s, int bits, uint32_t h, uint32_t w) {
   s.serial = ++serial_;
 }
 
+size_t Engine::slot_bytes(uint32_t height, uint32_t width, int bits) {
+  const size_t n = static_cast<size_t>(height) * width;
+  const size_t words = static_cast<size_t>(height) * mask_words_per_row(width);
+  const size_t cap = n * static_cast<size_t>(bits / 8) / sizeof(sf_signal_pixel);
+  // mask and prefix; two lists; parent and comp_of; components; links; the
+  // root bits and scans; runs per row.
+  return 8 * words + 32 * cap + 8 * (cap + 1) + sizeof(sf_component) * (cap + 1) +
+         8 * (cap + 1) + 8 * (cap / 32 + 2) + 4 * (static_cast<size_t>(height) + 1) +
+         (1u << 16);
+}
+
 void Engine::shape_label(FrameSlot &s) {
   const size_t cap = s.capacity;
   s.root_words = cap / 32 + 2;
@@ -361,9 +372,11 @@ void Engine::read_counters() {
 
 void Engine::sync(const int *ids, size_t n, size_t *counts, int *status) {
   read_counters();
+  size_t first_redone = n;
   for (size_t i = 0; i < n; i++) {
     FrameSlot &s = slot(ids[i]);
     if (host_counters_[kCounters * ids[i] + 1] != 0 && s.kernel == SF_KERNEL_FUSED) {
+      first_redone = first_redone < i ? first_redone : i;
       // A valid pixel beyond the fused kernel's exact range: this frame is
       // thresholded again, in its own slot, by the direct kernels, whose sums
       // are in the reference's types.
@@ -377,6 +390,22 @@ void Engine::sync(const int *ids, size_t n, size_t *counts, int *status) {
       read_counters();
     }
   }
+  // Labels enqueued before this sync were taken from the lists the exact path
+  // replaced: relabel the re-run frames and, since a frame's links read the
+  // components of the frame before, every labelled frame after the first.
+  bool relabelled = false;
+  for (size_t i = first_redone; i < n; i++) {
+    FrameSlot &s = slot(ids[i]);
+    if (!s.labelled)
+      continue;
+    int prev = s.linked_slot;
+    if (prev >= 0 && slot(prev).serial != s.linked_serial)
+      prev = -1;
+    label(&ids[i], &prev, 1);
+    relabelled = true;
+  }
+  if (relabelled)
+    read_counters();
   for (size_t i = 0; i < n; i++) {
     const uint32_t c = host_counters_[kCounters * ids[i]];
     if (counts)
@@ -465,11 +494,14 @@ void Engine::label(const int *ids, const int *prev, size_t n) {
     slot(ids[i]).labelled = true;
     slot(ids[i]).linked_serial =
         (prev && prev[i] >= 0) ? slot(prev[i]).serial : 0;
+    slot(ids[i]).linked_slot = (prev && prev[i] >= 0) ? prev[i] : -1;
   }
 }
 
-void Engine::label_counts(int id, size_t *n, size_t *ncomp, size_t *nlinks) {
-  read_counters();
+void Engine::label_counts(int id, size_t *n, size_t *ncomp, size_t *nlinks,
+                          bool fresh) {
+  if (fresh)
+    read_counters();
   const uint32_t *c = host_counters_ + kCounters * id;
   if (c[3] > slot(id).link_capacity)
     throw device_error("more frame links than the link list holds");

@@ -6,6 +6,7 @@
 // one stream and grow-only buffers per worker thread, the stream idle on
 // every exit path, -1 for bad arguments, -2 when the list would overflow,
 // and device failures reported with the runtime's own message.
+#include <algorithm>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -689,6 +690,110 @@ int sf_batch_kernel_ms(sf_batch *b, double *threshold_ms, double *order_ms) {
 
 uint64_t sf_batch_fallbacks(sf_batch *b) {
   return b ? b->engine.fallbacks() : 0;
+}
+
+// ---- the autotuner ---------------------------------------------------------
+
+int sf_autotune(const sf_frame_desc *frames, size_t n, int label,
+                sf_autotune_report *report) {
+  if (frames == nullptr || n == 0 || report == nullptr)
+    return bad("frames or report");
+  *report = sf_autotune_report{};
+  const sf_tuning start = tuning();
+  sf_batch *b = nullptr;
+  int rc = sf_batch_create(nullptr, &b);
+  if (rc != 0)
+    return rc;
+  std::unique_ptr<sf_batch, void (*)(sf_batch *)> own(b, sf_batch_destroy);
+  auto fail = [&](int code) {
+    (void)sf_set_tuning(&start);
+    return code;
+  };
+  const size_t shape_n = n < 128 ? n : 128;
+  std::vector<size_t> counts(n);
+  std::vector<int> status(n);
+
+  // 1. The fused kernel's shape, by its own duration over the first frames
+  //    (median of three).
+  sf_tuning best = start;
+  double best_ms = 0.0;
+  const int waves_per_block[] = {1, 2, 4};
+  const int seg_rows[] = {64, 128, 192, 256, 0, 0, 0};
+  const int target_waves[] = {start.target_waves, start.target_waves,
+                              start.target_waves, start.target_waves, 4096,
+                              8192, 16384};
+  sf_batch_set_timing(b, 1);
+  for (int w : waves_per_block)
+    for (size_t k = 0; k < sizeof(seg_rows) / sizeof(seg_rows[0]); k++) {
+      sf_tuning t = start;
+      t.waves_per_block = w;
+      t.seg_rows = seg_rows[k];
+      t.target_waves = target_waves[k];
+      if ((rc = sf_set_tuning(&t)) != 0)
+        return fail(rc);
+      double ms[3];
+      for (double &m : ms) {
+        if ((rc = sf_batch_threshold(b, frames, shape_n, 0)) != 0 ||
+            (rc = sf_batch_sync(b, counts.data(), status.data())) != 0 ||
+            (rc = sf_batch_kernel_ms(b, &m, nullptr)) != 0)
+          return fail(rc);
+      }
+      std::sort(ms, ms + 3);
+      report->shapes_tried++;
+      if (best_ms == 0.0 || ms[1] < best_ms) {
+        best_ms = ms[1];
+        best = t;
+      }
+    }
+  sf_batch_set_timing(b, 0);
+  report->fused_ms_per_frame = best_ms / static_cast<double>(shape_n);
+
+  // 2. Frames per launch, by the whole step (threshold, ordering and, with
+  //    `label`, labelling) per frame, over the sizes that fit: a batch holds
+  //    two sets of slots, so only sizes whose slots take at most 60% of the
+  //    free device memory are tried.
+  size_t slot = 0;
+  for (size_t i = 0; i < n; i++) {
+    const size_t s = Engine::slot_bytes(frames[i].height, frames[i].width,
+                                        frames[i].bits);
+    slot = s > slot ? s : slot;
+  }
+  const double room = 0.6 * static_cast<double>(sf_memory_free());
+  if ((rc = sf_set_tuning(&best)) != 0)
+    return fail(rc);
+  double best_step = 0.0;
+  int best_f = best.frames_per_launch;
+  for (int f : {16, 32, 64, 128}) {
+    if (static_cast<size_t>(f) > n ||
+        2.0 * static_cast<double>(f) * static_cast<double>(slot) > room)
+      continue;
+    const size_t launches = (n < 128 ? n : 128) / static_cast<size_t>(f);
+    double secs = 0.0;
+    for (int rep = 0; rep < 2; rep++) { // the first pass sizes the slots
+      const auto t0 = std::chrono::steady_clock::now();
+      for (size_t l = 0; l < launches; l++) {
+        if ((rc = sf_batch_threshold(b, frames + l * f, static_cast<size_t>(f),
+                                     1)) != 0 ||
+            (label && (rc = sf_batch_label(b, 1)) != 0) ||
+            (rc = sf_batch_sync(b, counts.data(), status.data())) != 0)
+          return fail(rc);
+      }
+      secs = std::chrono::duration<double>(std::chrono::steady_clock::now() - t0)
+                 .count();
+    }
+    const double per = 1e3 * secs / static_cast<double>(launches * f);
+    report->launch_sizes_tried++;
+    if (best_step == 0.0 || per < best_step) {
+      best_step = per;
+      best_f = f;
+    }
+  }
+  best.frames_per_launch = best_f;
+  report->step_ms_per_frame = best_step;
+  report->best = best;
+  if ((rc = sf_set_tuning(&best)) != 0)
+    return fail(rc);
+  return 0;
 }
 
 // ---- grouping --------------------------------------------------------------

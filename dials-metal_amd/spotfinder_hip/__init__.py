@@ -64,6 +64,44 @@ class Tuning(ctypes.Structure):
         return {f: getattr(self, f) for f, _ in self._fields_}
 
 
+class AutotuneReport(ctypes.Structure):
+    _fields_ = [("best", Tuning), ("fused_ms_per_frame", ctypes.c_double),
+                ("step_ms_per_frame", ctypes.c_double),
+                ("shapes_tried", ctypes.c_uint32),
+                ("launch_sizes_tried", ctypes.c_uint32)]
+
+    def as_dict(self):
+        d = {f: getattr(self, f) for f, _ in self._fields_ if f != "best"}
+        d["best"] = self.best.as_dict()
+        return d
+
+
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("max_batch", ctypes.c_uint32), ("ring", ctypes.c_uint32),
+                ("label", ctypes.c_int32), ("reserve_bytes", ctypes.c_uint64)]
+
+
+class ServerResult(ctypes.Structure):
+    _fields_ = [("request", ctypes.c_uint64), ("sweep", ctypes.c_int32),
+                ("status", ctypes.c_int32), ("signal_pixels", ctypes.c_uint64),
+                ("components", ctypes.c_uint64), ("links", ctypes.c_uint64),
+                ("latency_ms", ctypes.c_double),
+                ("batch_frames", ctypes.c_uint32), ("reserved", ctypes.c_uint32)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_ if f != "reserved"}
+
+
+class ServerStats(ctypes.Structure):
+    _fields_ = [("frames", ctypes.c_uint64), ("batches", ctypes.c_uint64),
+                ("p50_ms", ctypes.c_double), ("p99_ms", ctypes.c_double),
+                ("max_ms", ctypes.c_double), ("mean_ms", ctypes.c_double),
+                ("frames_per_s", ctypes.c_double), ("mean_batch", ctypes.c_double)]
+
+    def as_dict(self):
+        return {f: getattr(self, f) for f, _ in self._fields_}
+
+
 class SpotfinderError(RuntimeError):
     """A status other than success from the C ABI."""
 
@@ -180,6 +218,21 @@ def lib():
         "sf_labeller_pixels": ([vp], vp),
         "sf_labeller_counts": ([vp], Counts),
         "sf_labeller_live": ([vp], sz),
+        "sf_autotune": ([vp, sz, ctypes.c_int, ctypes.POINTER(AutotuneReport)],
+                        ctypes.c_int),
+        "sf_server_config_default": ([ctypes.POINTER(ServerConfig)], None),
+        "sf_server_create": ([ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],
+                             ctypes.c_int),
+        "sf_server_destroy": ([vp], None),
+        "sf_server_open_sweep": ([vp, u32, u32, ctypes.c_int,
+                                  ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "sf_server_submit": ([vp, ctypes.c_int, vp, vp, sz,
+                              ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+        "sf_server_wait": ([vp, ctypes.c_uint64, ctypes.POINTER(ServerResult)],
+                           ctypes.c_int),
+        "sf_server_stats_get": ([vp, ctypes.POINTER(ServerStats)], ctypes.c_int),
+        "sf_server_stats_reset": ([vp], None),
+        "sf_server_replay": ([vp, vp, vp, vp, sz, vp], ctypes.c_int),
         "sf_synth_default": ([ctypes.POINTER(SynthDesc), u32, u32], None),
         "sf_synth_eiger": ([ctypes.POINTER(SynthDesc), ctypes.c_int, ctypes.c_int], None),
         "sf_synth_frame": ([ctypes.POINTER(SynthDesc), ctypes.c_int64, vp], ctypes.c_int),
@@ -569,6 +622,85 @@ class Labeller:
 
     def live(self):
         return int(lib().sf_labeller_live(self.h))
+
+
+def autotune(frames, label=True):
+    """sf_autotune over device frames given as (ptr, height, width, bits):
+    measures the fused kernel's shape and frames per launch, installs the
+    best, and returns the report as a dict."""
+    arr = (FrameDesc * len(frames))(*[FrameDesc(p, h, w, b) for p, h, w, b in frames])
+    rep = AutotuneReport()
+    _check(lib().sf_autotune(ctypes.cast(arr, ctypes.c_void_p), len(frames),
+                             int(bool(label)), ctypes.byref(rep)))
+    return rep.as_dict()
+
+
+class NotAdmitted(SpotfinderError):
+    """A server refused a sweep: too little free device memory."""
+
+
+class Server:
+    """The serving loop (sf_server_*): device-resident frames from several
+    sweeps, batched as they arrive by a worker thread of the library."""
+
+    def __init__(self, max_batch=64, ring=4, label=True, reserve_bytes=1 << 30):
+        cfg = ServerConfig(max_batch, ring, int(bool(label)), reserve_bytes)
+        self.h = ctypes.c_void_p()
+        _check(lib().sf_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def close(self):
+        if self.h:
+            lib().sf_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def open_sweep(self, height, width, bits=32):
+        s = ctypes.c_int(-1)
+        rc = lib().sf_server_open_sweep(self.h, height, width, bits, ctypes.byref(s))
+        if rc == -6:
+            raise NotAdmitted(rc, last_error())
+        _check(rc)
+        return s.value
+
+    def submit(self, sweep, dev_ptr, out=None):
+        """Queues a device frame; with `out` (a SIGNAL_PIXEL array) its ordered
+        list is copied there before the request completes."""
+        r = ctypes.c_uint64(0)
+        _check(lib().sf_server_submit(self.h, sweep, dev_ptr,
+                                      None if out is None else _ptr(out),
+                                      0 if out is None else len(out),
+                                      ctypes.byref(r)))
+        return r.value
+
+    def wait(self, request):
+        res = ServerResult()
+        _check(lib().sf_server_wait(self.h, request, ctypes.byref(res)))
+        return res.as_dict()
+
+    def stats(self):
+        st = ServerStats()
+        _check(lib().sf_server_stats_get(self.h, ctypes.byref(st)))
+        return st.as_dict()
+
+    def reset_stats(self):
+        lib().sf_server_stats_reset(self.h)
+
+    def replay(self, sweeps, frames, arrivals):
+        """Submits frames[i] (device pointers) for sweeps[i] at arrivals[i]
+        seconds (ascending), waits for all; the results in order."""
+        n = len(sweeps)
+        sw = np.ascontiguousarray(sweeps, np.int32)
+        fr = np.ascontiguousarray(frames, np.uint64)
+        ar = np.ascontiguousarray(arrivals, np.float64)
+        out = (ServerResult * n)()
+        _check(lib().sf_server_replay(self.h, _ptr(sw), _ptr(fr), _ptr(ar), n,
+                                      ctypes.cast(out, ctypes.c_void_p)))
+        return [r.as_dict() for r in out]
 
 
 def workspace_label_enqueue(ws, link_previous=True):

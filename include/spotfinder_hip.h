@@ -12,8 +12,9 @@
  * Status codes: 0 success; -1 bad argument; -2 more signal pixels than the
  * device list holds (nothing is returned, as dext_cuda.cu:821-822); -3 a
  * device or runtime failure, with the runtime's message in sf_last_error();
- * -4 out of memory; -5 unsupported request.  sf_last_error() describes the
- * last failure on the calling thread.
+ * -4 out of memory; -5 unsupported request; -6 not admitted (a server has
+ * no room for another sweep).  sf_last_error() describes the last failure
+ * on the calling thread.
  */
 #ifndef SPOTFINDER_HIP_H
 #define SPOTFINDER_HIP_H
@@ -252,6 +253,24 @@ void sf_batch_set_timing(sf_batch *b, int on);
 int sf_batch_kernel_ms(sf_batch *b, double *threshold_ms, double *order_ms);
 uint64_t sf_batch_fallbacks(sf_batch *b);
 
+/* The autotuner.  No counterpart in the reference, whose launch shape is fixed
+ * (dext_cuda.cu:728-730).  On the caller's own device-resident frames it
+ * measures the fused kernel's shape -- waves per block {1, 2, 4} by segment
+ * sizing {64, 128, 192, 256 rows, or automatic at 4096/8192/16384 target
+ * waves} -- by the kernel's own duration over the first (up to 128) frames,
+ * then frames per launch {16, 32, 64, 128} by the whole step per frame
+ * (threshold, ordering and, with `label`, labelling) among the sizes whose
+ * slots fit in 60% of free device memory, and installs the winner with
+ * sf_set_tuning.  On failure the tuning is left as it was. */
+typedef struct sf_autotune_report {
+  sf_tuning best;
+  double fused_ms_per_frame; /* the best shape's kernel time per frame */
+  double step_ms_per_frame;  /* the best launch size's step time per frame */
+  uint32_t shapes_tried, launch_sizes_tried;
+} sf_autotune_report;
+int sf_autotune(const sf_frame_desc *frames, size_t n, int label,
+                sf_autotune_report *report);
+
 /* ------------------------------------------------------------------------
  * Grouping into spots: dials_spots::Labeller (src/spots/dials_spots.hh:
  * 108-158) with the pixel-level labelling and reductions on the device.
@@ -300,6 +319,70 @@ size_t sf_labeller_pixel_count(const sf_labeller *l);
 const sf_pixel *sf_labeller_pixels(const sf_labeller *l);
 sf_counts sf_labeller_counts(const sf_labeller *l);
 size_t sf_labeller_live(const sf_labeller *l);
+
+/* ------------------------------------------------------------------------
+ * Serving: frames from several concurrent sweeps, as they arrive.  No
+ * counterpart in the reference, whose mxi_find runs one sweep from files
+ * (find_spots.cc:732-891); this is the same per-frame work (the threshold,
+ * the ordered list, 2D components linked to the sweep's previous frame)
+ * behind a request loop.  A worker thread takes every frame that has
+ * arrived (at most ring - 1 per sweep) into one batch -- one fused launch
+ * for all of them -- while later arrivals wait for the next: continuous
+ * batching.  Each sweep keeps its own ring of frame slots, so its label
+ * scratch stays resident and a frame links to its sweep's previous frame.
+ * A sweep is admitted only while free HBM stays above the reserve.
+ * ------------------------------------------------------------------------ */
+typedef struct sf_server sf_server;
+
+typedef struct sf_server_config {
+  uint32_t max_batch; /* frames per launch, all sweeps together (0: 64) */
+  uint32_t ring;      /* slots per sweep, >= 2 (0: 4) */
+  int32_t label;      /* 1: label every frame (2D, linked to its sweep) */
+  uint64_t reserve_bytes; /* HBM to leave free when admitting a sweep */
+} sf_server_config;
+
+typedef struct sf_server_result {
+  uint64_t request;
+  int32_t sweep;
+  int32_t status;          /* 0, or -2 (more signal than the list holds) */
+  uint64_t signal_pixels;
+  uint64_t components;     /* 0 unless labelling */
+  uint64_t links;          /* frame links to the sweep's previous frame */
+  double latency_ms;       /* submission to results ready */
+  uint32_t batch_frames;   /* frames in the launch that served it */
+  uint32_t reserved;
+} sf_server_result;
+
+typedef struct sf_server_stats {
+  uint64_t frames;         /* completed since the last reset */
+  uint64_t batches;
+  double p50_ms, p99_ms, max_ms, mean_ms; /* latency, submission to ready */
+  double frames_per_s;     /* completed frames over first submission .. last completion */
+  double mean_batch;
+} sf_server_stats;
+
+void sf_server_config_default(sf_server_config *c);
+int sf_server_create(const sf_server_config *c, sf_server **out);
+void sf_server_destroy(sf_server *s);
+/* -6 when admitting the sweep would leave less than the reserve free. */
+int sf_server_open_sweep(sf_server *s, uint32_t height, uint32_t width,
+                         int bits, int *sweep);
+/* Queues a device-resident frame of the sweep's shape, stamped with the
+ * time of the call.  With host_out, its ordered signal list (at most
+ * host_capacity entries) is copied there before the request completes. */
+int sf_server_submit(sf_server *s, int sweep, const void *device_frame,
+                     sf_signal_pixel *host_out, size_t host_capacity,
+                     uint64_t *request);
+/* Waits for one request. */
+int sf_server_wait(sf_server *s, uint64_t request, sf_server_result *result);
+int sf_server_stats_get(sf_server *s, sf_server_stats *stats);
+void sf_server_stats_reset(sf_server *s);
+/* A load driver for measurement: submits frame i of `frames` for sweep
+ * sweeps[i] at arrival_s[i] seconds after the call (ascending), from a
+ * thread of its own, then waits for all; results in submission order. */
+int sf_server_replay(sf_server *s, const int *sweeps,
+                     const void *const *frames, const double *arrival_s,
+                     size_t n, sf_server_result *results);
 
 #ifdef __cplusplus
 }

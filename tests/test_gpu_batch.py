@@ -140,3 +140,32 @@ def test_batch_label_group(device, mode):
     assert lab.pixels().tobytes() == ref.pixels().tobytes()
     assert lab.counts() == ref.counts()
     batch.close()
+
+
+def test_autotune_installs_a_shape_and_results_do_not_change(device):
+    """sf_autotune measures and installs a launch shape; whatever it picks,
+    every frame's list is the same as under the starting shape."""
+    desc = device.synth_desc("4m", spots_per_frame=500, persist=2, seed=41)
+    host = [device.synth_frame(desc, z) for z in range(16)]
+    dev = [device.DeviceArray.from_numpy(f) for f in host]
+    descs = [(d.ptr, desc.height, desc.width, 32) for d in dev]
+    start = device.get_tuning()
+    batch = device.Batch()
+    batch.threshold(descs)
+    counts, status = batch.sync()
+    before = [batch.fetch(i, counts[i]) for i in range(len(descs))]
+    try:
+        say("autotune")
+        report = device.autotune(descs, label=True)
+        say(report)
+        assert report["shapes_tried"] == 21
+        assert report["launch_sizes_tried"] >= 1
+        assert report["best"] == device.get_tuning()
+        assert report["fused_ms_per_frame"] > 0 and report["step_ms_per_frame"] > 0
+        batch.threshold(descs)
+        counts2, status2 = batch.sync()
+        assert list(counts2) == list(counts) and list(status2) == list(status)
+        for i in range(len(descs)):
+            assert batch.fetch(i, counts2[i]).tobytes() == before[i].tobytes(), i
+    finally:
+        device.set_tuning(**start)

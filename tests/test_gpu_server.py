@@ -1,0 +1,107 @@
+"""The serving loop (sf_server_*) on the device.
+
+  * alone and in the mix: the frames of three sweeps (two EIGER2 4M uint32,
+    one 512 x 512 uint16 ROI) served one at a time, then all submitted at
+    once so that the worker batches them together across sweeps; every
+    frame's ordered list, signal count, component count and link count are
+    identical both ways, the lists equal the oracle's threshold, and the mix
+    really was served in shared batches;
+  * admission: a sweep that would leave less free device memory than the
+    reserve is refused (-6);
+  * a request can be waited for once.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SWEEPS = [("4m", dict(seed=31, spots_per_frame=600, persist=2)),
+          ("4m", dict(seed=32, spots_per_frame=300, persist=1)),
+          ((512, 512), dict(bits=16, seed=33, spots_per_frame=40,
+                            ring_radius=150.0))]
+FRAMES = 4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+@pytest.fixture(scope="module")
+def data(device):
+    out = []
+    for kind, kw in SWEEPS:
+        d = device.synth_desc(kind, **kw)
+        frames = [device.synth_frame(d, z) for z in range(FRAMES)]
+        out.append((d, frames, [device.DeviceArray.from_numpy(f) for f in frames]))
+    return out
+
+
+def capacity(device, d):
+    return d.height * d.width * (d.bits // 8) // 16
+
+
+def serve(device, data, together):
+    srv = device.Server(max_batch=64, ring=FRAMES + 1, label=True)
+    ids = [srv.open_sweep(d.height, d.width, d.bits) for d, _, _ in data]
+    results = {}
+    order = [(s, z) for z in range(FRAMES) for s in range(len(data))]
+    pending = []
+    for s, z in order:
+        d, _, dev = data[s]
+        out = np.zeros(capacity(device, d), device.SIGNAL_PIXEL)
+        r = srv.submit(ids[s], dev[z].ptr, out)
+        if together:
+            pending.append((s, z, r, out))
+        else:
+            res = srv.wait(r)
+            results[(s, z)] = (res, out[: res["signal_pixels"]].copy())
+    for s, z, r, out in pending:
+        res = srv.wait(r)
+        results[(s, z)] = (res, out[: res["signal_pixels"]].copy())
+    stats = srv.stats()
+    srv.close()
+    return results, stats
+
+
+def test_each_result_is_the_same_alone_and_in_the_mix(device, data, oracle_lib):
+    say("server: alone")
+    alone, st_alone = serve(device, data, together=False)
+    say("server: in the mix", st_alone)
+    mixed, st_mixed = serve(device, data, together=True)
+    say("server: mixed", st_mixed)
+    assert st_alone["mean_batch"] == 1.0
+    assert st_mixed["mean_batch"] > 1.0 and st_mixed["batches"] < len(mixed)
+    for key, (res_a, list_a) in alone.items():
+        res_m, list_m = mixed[key]
+        s, z = key
+        assert res_a["status"] == 0 and res_m["status"] == 0, key
+        for field in ("signal_pixels", "components", "links"):
+            assert res_a[field] == res_m[field], (key, field)
+        assert list_a.tobytes() == list_m.tobytes(), key
+        _, want = oracle_lib.dext(data[s][1][z])
+        assert list_a.tobytes() == want.tobytes(), key
+        # Where spots persist over frames, every frame after a sweep's first
+        # links to its previous frame.
+        if z > 0 and data[s][0].persist >= 2:
+            assert res_a["links"] > 0, key
+
+
+def test_admission_by_free_memory(device):
+    srv = device.Server(reserve_bytes=1 << 62)
+    with pytest.raises(device.NotAdmitted):
+        srv.open_sweep(4362, 4148, 32)
+    srv.close()
+
+
+def test_a_request_is_waited_for_once(device, data):
+    srv = device.Server(label=False)
+    d, _, dev = data[2]
+    sweep = srv.open_sweep(d.height, d.width, d.bits)
+    r = srv.submit(sweep, dev[0].ptr)
+    assert srv.wait(r)["status"] == 0
+    with pytest.raises(device.SpotfinderError):
+        srv.wait(r)
+    srv.close()
