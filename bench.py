@@ -34,6 +34,24 @@ METRIC = ("16M frames/s + achieved HBM GB/s (bit-exact mask); "
           "serving p50/p99 frame latency")
 
 
+TRAFFIC = os.path.join(ROOT, "profiles", "fused_kernel_16m_batch32_traffic.json")
+
+
+def measured_traffic(frames, tuning):
+    """HBM bytes per launch of the fused kernel from the committed rocprofv3
+    PMC measurement (FETCH_SIZE and WRITE_SIZE scaled by a device copy of
+    known size), when it was taken at this batch size and launch shape."""
+    try:
+        with open(TRAFFIC) as fh:
+            m = json.load(fh)
+    except (OSError, ValueError):
+        return None, None
+    if m.get("frames_per_launch") != frames or any(
+            tuning.get(k) != v for k, v in m.get("tuning", {}).items()):
+        return None, None
+    return m["traffic_bytes_per_launch"], "profiles/" + os.path.basename(TRAFFIC)
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -256,6 +274,7 @@ def main():
     n_px = h * w
     bytes_per_frame = n_px * 4 + 16 * mean_signal + n_px / 8.0
     achieved = len(dev) * bytes_per_frame / (kernel_ms * 1e-3) / 1e9
+    traffic, traffic_source = measured_traffic(len(dev), sf.get_tuning())
 
     line = {
         "metric": METRIC,
@@ -282,7 +301,8 @@ def main():
             "mean_signal_pixels": mean_signal},
         "roofline": {
             "bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+            "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+            "traffic_source": traffic_source,
             "kernel": "fused_kernel (one launch per batch)",
             "ms_per_launch": kernel_ms,
             "bytes_per_frame": bytes_per_frame,
