@@ -20,6 +20,7 @@ struct FrameSlot {
   DeviceBuffer mask, prefix, totals, unordered, ordered;
   DeviceBuffer parent, roots, root_prefix, root_totals, comp_of, comps, links;
   DeviceBuffer row_runs; // runs per bitmask row, then each row's first run
+  DeviceBuffer comp_runs; // runs per component, then each component's first
   uint32_t height = 0, width = 0, wpr = 0, capacity = 0;
   int bits = 0;
   size_t words = 0, root_words = 0;

@@ -146,7 +146,7 @@ size_t Engine::slot_bytes(uint32_t height, uint32_t width, int bits) {
   const size_t cap = n * static_cast<size_t>(bits / 8) / sizeof(sf_signal_pixel);
   // mask and prefix; two lists; parent and comp_of; components; links; the
   // root bits and scans; runs per row.
-  return 8 * words + 32 * cap + 8 * (cap + 1) + sizeof(sf_component) * (cap + 1) +
+  return 8 * words + 32 * cap + 12 * (cap + 1) + sizeof(sf_component) * (cap + 1) +
          8 * (cap + 1) + 8 * (cap / 32 + 2) + 4 * (static_cast<size_t>(height) + 1) +
          (1u << 16);
 }
@@ -159,6 +159,7 @@ void Engine::shape_label(FrameSlot &s) {
   grow(s.root_prefix, s.root_words * 4);
   grow(s.root_totals, scan_tmp_words(s.root_words) * 4);
   grow(s.row_runs, (static_cast<size_t>(s.height) + 1) * 4);
+  grow(s.comp_runs, (cap + 2) * 4);
   grow(s.comp_of, (cap + 1) * 4);
   grow(s.comps, (cap + 1) * sizeof(sf_component));
   s.link_capacity = static_cast<uint32_t>(cap + 1);
@@ -183,6 +184,7 @@ FrameDev Engine::dev(int id, int prev) {
   f.root_prefix = s.root_prefix.as<uint32_t>();
   f.comp_of = s.comp_of.as<uint32_t>();
   f.row_runs = s.row_runs.as<uint32_t>();
+  f.comp_runs = s.comp_runs.as<uint32_t>();
   f.comps = s.comps.as<sf_component>();
   f.links = s.links.as<uint32_t>();
   f.link_capacity = s.link_capacity;

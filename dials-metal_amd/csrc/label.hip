@@ -20,9 +20,11 @@
 //     component is its lowest rank, i.e. its first pixel;
 //   * roots are numbered by a popcount scan of a root bitmask, which numbers
 //     components in order of first appearance -- the reference's order;
-//   * each run sums its pixels' exact integer moments in registers and adds
-//     them to its component with one set of atomics (128-bit where a product
-//     can exceed 64 bits), and writes its pixels' component.
+//   * the runs are sorted by component (one atomic per run takes its place,
+//     a scan of the counts gives each component's first), and one thread per
+//     component sums its runs' pixels' exact integer moments in registers
+//     (128-bit where a product can exceed 64 bits) and writes the record whole
+//     -- no atomics on the records -- and its pixels' component.
 // Links to the previous frame (pixels at the same position in both frames)
 // are listed per run, as the components of the previous frame's bitmask
 // segments under the run, for the host's three-dimensional stitch
@@ -311,8 +313,8 @@ __device__ __forceinline__ uint32_t component_of(const FrameDev &f, uint32_t r) 
          static_cast<uint32_t>(__popc(gmem(f.roots)[r / 32] & ((1u << (r & 31u)) - 1u)));
 }
 
-// Each root run opens its component's record; the run holding the last rank
-// writes the component count.
+// Each root run opens its component (its run count starts at zero); the run
+// holding the last rank writes the component count.
 __global__ __launch_bounds__(256) void open_components(
     const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
@@ -323,25 +325,80 @@ __global__ __launch_bounds__(256) void open_components(
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
        k += gridDim.x * blockDim.x) {
     const run4 r = runs[k];
-    if (parent[r.x] == r.x) {
-      sf_component z{};
-      z.first = r.z * f.width + r.y;
-      z.x0 = z.y0 = 0xFFFFFFFFu;
-      f.comps[component_of(f, r.x)] = z;
-    }
+    if (parent[r.x] == r.x)
+      gmem(f.comp_runs)[component_of(f, r.x)] = 0;
     if (r.x + r.w == n)
       f.counter[2] = component_of(f, n - 1) + (parent[n - 1] == n - 1 ? 1u : 0u);
   }
 }
 
-__device__ __forceinline__ void add128(uint64_t *pair, uint64_t lo, uint64_t hi) {
-  const unsigned long long old =
-      atomicAdd(reinterpret_cast<unsigned long long *>(&pair[0]),
-                static_cast<unsigned long long>(lo));
-  const uint64_t carry = (static_cast<uint64_t>(old) + lo) < lo ? 1u : 0u;
-  if (hi + carry)
-    atomicAdd(reinterpret_cast<unsigned long long *>(&pair[1]),
-              static_cast<unsigned long long>(hi + carry));
+// Each run takes a place among its component's runs, one atomic per run (a
+// component's few runs contend, nothing else does); its component and place
+// wait in the link list's memory, which is free until the links are listed.
+__global__ __launch_bounds__(256) void count_component_runs(
+    const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t nr = f.counter[4];
+  const SF_GLOBAL uint32_t *parent = gmem(f.parent);
+  const SF_GLOBAL run4 *runs = run_list(f);
+  SF_GLOBAL uint32_t *scratch = gmem(f.links);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
+       k += gridDim.x * blockDim.x) {
+    const uint32_t c = component_of(f, parent[runs[k].x]);
+    scratch[2 * k] = c;
+    scratch[2 * k + 1] = __hip_atomic_fetch_add(gmem(f.comp_runs) + c, 1u,
+                                                __ATOMIC_RELAXED,
+                                                __HIP_MEMORY_SCOPE_AGENT);
+  }
+}
+
+// Each component's first run in component order -- an exclusive scan of the
+// runs per component, in place -- and the run count after the last; one block
+// of 1024 threads per frame, each thread a contiguous share.
+__global__ __launch_bounds__(1024) void scan_components(
+    const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.x];
+  __shared__ uint32_t wave_sums[1024 / kLanes];
+  const uint32_t nc = f.counter[2];
+  SF_GLOBAL uint32_t *counts = gmem(f.comp_runs);
+  const uint32_t per = (nc + blockDim.x - 1) / blockDim.x;
+  const uint32_t c0 = threadIdx.x * per;
+  const uint32_t c1 = min(nc, c0 + per);
+  uint32_t mine = 0;
+  for (uint32_t c = c0; c < c1; c++)
+    mine += counts[c];
+  const uint32_t lane = threadIdx.x % kLanes, wave = threadIdx.x / kLanes;
+  uint32_t total;
+  uint32_t before = wave_prefix(mine, lane, &total);
+  if (lane == 0)
+    wave_sums[wave] = total;
+  __syncthreads();
+  uint32_t all = 0;
+  for (uint32_t w = 0; w < blockDim.x / kLanes; w++) {
+    if (w < wave)
+      before += wave_sums[w];
+    all += wave_sums[w];
+  }
+  for (uint32_t c = c0; c < c1; c++) {
+    const uint32_t k = counts[c];
+    counts[c] = before;
+    before += k;
+  }
+  if (threadIdx.x == 0)
+    counts[nc] = all;
+}
+
+// The runs in component order, in the parent array's memory (the forest is
+// not needed once every run knows its component).
+__global__ __launch_bounds__(256) void place_runs(const FrameDev *__restrict__ frames) {
+  const FrameDev f = frames[blockIdx.y];
+  const uint32_t nr = f.counter[4];
+  const SF_GLOBAL uint32_t *scratch = gmem(f.links);
+  const SF_GLOBAL uint32_t *first = gmem(f.comp_runs);
+  SF_GLOBAL uint32_t *order = gmem(f.parent);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
+       k += gridDim.x * blockDim.x)
+    order[first[scratch[2 * k]] + scratch[2 * k + 1]] = k;
 }
 
 // Adds a 128-bit value held as (lo, hi) to a running (lo, hi) in registers.
@@ -352,54 +409,54 @@ __device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t l,
   lo = s;
 }
 
-// Exact moments into the components, one run at a time: the run's pixels are
-// consecutive ranks of the ordered list, summed in registers, then added to
-// the component with one set of atomics; every pixel's component is written.
+// Exact moments, one thread per component: its runs (in component order) and
+// their pixels (consecutive ranks of the ordered list) summed in registers,
+// the record written whole -- no atomics -- and every pixel's component
+// written.
 __global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
-  const uint32_t nr = f.counter[4];
-  const SF_GLOBAL uint32_t *parent = gmem(f.parent);
+  const uint32_t nc = f.counter[2];
+  const SF_GLOBAL uint32_t *first_run = gmem(f.comp_runs);
+  const SF_GLOBAL uint32_t *order = gmem(f.parent);
   const SF_GLOBAL run4 *runs = run_list(f);
   const SF_GLOBAL sf_signal_pixel *ordered = gmem(f.ordered);
   SF_GLOBAL uint32_t *comp_of = gmem(f.comp_of);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
-       k += gridDim.x * blockDim.x) {
-    const run4 r = runs[k]; // r0, x0, y, length
-    const uint32_t c = component_of(f, parent[r.x]);
-    const uint32_t first = r.z * f.width + r.y;
-    const uint64_t ay = 2ull * r.z + 1ull;
-    uint64_t peak = 0, sum = 0;
-    uint64_t sq_lo = 0, sq_hi = 0, sx_lo = 0, sx_hi = 0, xx_lo = 0, xx_hi = 0;
-    for (uint32_t j = 0; j < r.w; j++) {
-      const uint64_t v = ordered[r.x + j].value;
-      comp_of[r.x + j] = c;
-      const uint64_t ax = 2ull * (r.y + j) + 1ull;
-      const uint64_t key = (v << 32) | (0xFFFFFFFFu - (first + j));
-      peak = key > peak ? key : peak;
-      sum += v;
-      acc128(sq_lo, sq_hi, v * v, 0);
-      acc128(sx_lo, sx_hi, v * ax, 0);
-      const uint64_t xx = ax * ax;
-      acc128(xx_lo, xx_hi, v * xx, __umul64hi(v, xx));
+  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc;
+       c += gridDim.x * blockDim.x) {
+    sf_component z{};
+    z.first = z.x0 = z.y0 = 0xFFFFFFFFu;
+    const uint32_t end = first_run[c + 1];
+    for (uint32_t j = first_run[c]; j < end; j++) {
+      const run4 r = runs[order[j]]; // r0, x0, y, length
+      const uint32_t start = r.z * f.width + r.y;
+      z.first = min(z.first, start);
+      z.x0 = min(z.x0, r.y);
+      z.x1 = max(z.x1, r.y + r.w);
+      z.y0 = min(z.y0, r.z);
+      z.y1 = max(z.y1, r.z + 1);
+      z.n += r.w;
+      uint64_t sum = 0;
+      for (uint32_t p = 0; p < r.w; p++) {
+        const uint64_t v = ordered[r.x + p].value;
+        comp_of[r.x + p] = c;
+        const uint64_t ax = 2ull * (r.y + p) + 1ull;
+        const uint64_t key = (v << 32) | (0xFFFFFFFFu - (start + p));
+        z.peak = key > z.peak ? key : z.peak;
+        sum += v;
+        acc128(z.sum_sq[0], z.sum_sq[1], v * v, 0);
+        acc128(z.sum_x[0], z.sum_x[1], v * ax, 0);
+        const uint64_t xx = ax * ax;
+        acc128(z.sum_xx[0], z.sum_xx[1], v * xx, __umul64hi(v, xx));
+      }
+      z.sum += sum;
+      // sum of value * ay and value * ay^2 over the run: ay is the run's
+      // constant, so these are products of the run's sum.
+      const uint64_t ay = 2ull * r.z + 1ull;
+      acc128(z.sum_y[0], z.sum_y[1], sum * ay, __umul64hi(sum, ay));
+      const uint64_t yy = ay * ay;
+      acc128(z.sum_yy[0], z.sum_yy[1], sum * yy, __umul64hi(sum, yy));
     }
-    sf_component *cp = &f.comps[c];
-    atomicAdd(&cp->n, r.w);
-    atomicMin(&cp->x0, r.y);
-    atomicMax(&cp->x1, r.y + r.w);
-    atomicMin(&cp->y0, r.z);
-    atomicMax(&cp->y1, r.z + 1);
-    atomicMax(reinterpret_cast<unsigned long long *>(&cp->peak),
-              static_cast<unsigned long long>(peak));
-    atomicAdd(reinterpret_cast<unsigned long long *>(&cp->sum),
-              static_cast<unsigned long long>(sum));
-    add128(cp->sum_sq, sq_lo, sq_hi);
-    add128(cp->sum_x, sx_lo, sx_hi);
-    // sum of value * ay and value * ay^2 over the run: ay is the run's
-    // constant, so these are products of the run's sum.
-    add128(cp->sum_y, sum * ay, __umul64hi(sum, ay));
-    const uint64_t yy = ay * ay;
-    add128(cp->sum_yy, sum * yy, __umul64hi(sum, yy));
-    add128(cp->sum_xx, xx_lo, xx_hi);
+    f.comps[c] = z;
   }
 }
 
@@ -509,6 +566,9 @@ void launch_label(const FrameDev *frames_device, const ScanSeg *root_segs_device
   flatten<<<grid, 256, 0, stream>>>(frames_device);
   launch_popcount_scan(root_segs_device, nframes, max_root_words, stream);
   open_components<<<grid, 256, 0, stream>>>(frames_device);
+  count_component_runs<<<grid, 256, 0, stream>>>(frames_device);
+  scan_components<<<nframes, 1024, 0, stream>>>(frames_device);
+  place_runs<<<grid, 256, 0, stream>>>(frames_device);
   accumulate<<<grid, 256, 0, stream>>>(frames_device);
   link_frames<<<grid, 256, 0, stream>>>(frames_device);
   check(hipGetLastError(), "launching the labelling");

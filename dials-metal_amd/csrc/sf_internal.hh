@@ -178,6 +178,9 @@ struct FrameDev {
   // labelling
   uint32_t *parent, *roots, *root_prefix, *comp_of;
   uint32_t *row_runs; // height words: runs per row, then each row's first run
+  // components + 1 words: runs per component, then each component's first
+  // run in component order (and the run count after the last)
+  uint32_t *comp_runs;
   sf_component *comps;
   // the previous frame of the same shape, for three-dimensional links; its
   // counters say whether it was labelled (a refused, overfull frame has a
