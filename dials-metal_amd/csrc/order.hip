@@ -61,20 +61,24 @@ __device__ __forceinline__ size_t live_words(const sfi::ScanSeg &s) {
   return w < s.words ? w : s.words;
 }
 
-// Per-block totals of popcounts; block (x, frame).
+// Per-block totals of popcounts; block (x, frame).  Thread t reads words t,
+// t + 256, ... of the block's share, so that each load instruction of a wave
+// covers 256 consecutive bytes.
 __global__ __launch_bounds__(kScanThreads) void popcount_totals(
     const sfi::ScanSeg *__restrict__ segs) {
   const sfi::ScanSeg s = segs[blockIdx.y];
   const size_t words = live_words(s);
-  if (blockIdx.x * static_cast<size_t>(kScanPerBlock) >= s.words)
+  const size_t base = static_cast<size_t>(blockIdx.x) * kScanPerBlock;
+  if (base >= s.words)
     return;
-  const size_t base = static_cast<size_t>(blockIdx.x) * kScanPerBlock +
-                      static_cast<size_t>(threadIdx.x) * kPerThread;
+  const SF_GLOBAL uint32_t *bits = sfd::gmem(s.bits);
   uint32_t c = 0;
 #pragma unroll
-  for (int i = 0; i < kPerThread; i++)
-    if (base + i < words)
-      c += static_cast<uint32_t>(__popc(s.bits[base + i]));
+  for (int i = 0; i < kPerThread; i++) {
+    const size_t w = base + static_cast<size_t>(i) * kScanThreads + threadIdx.x;
+    if (w < words)
+      c += static_cast<uint32_t>(__popc(bits[w]));
+  }
   uint32_t total;
   (void)block_inclusive(c, &total);
   if (threadIdx.x == 0)
@@ -99,29 +103,52 @@ __global__ __launch_bounds__(kScanThreads) void scan_totals(
   }
 }
 
-// Exclusive prefix of popcounts per word, frame-local.
+// Exclusive prefix of popcounts per word, frame-local.  Each wave takes 1024
+// consecutive words of the block's 4096, 64 at a time and coalesced (lane l
+// reads word 64 i + l of its share): a wave scan orders each step's counts
+// and a running count carries from step to step; the waves' starting counts
+// come from their totals, exchanged in LDS.
 __global__ __launch_bounds__(kScanThreads) void popcount_prefix(
     const sfi::ScanSeg *__restrict__ segs) {
+  constexpr uint32_t kWaveWords = kScanPerBlock / (kScanThreads / 64);
+  __shared__ uint32_t wave_totals[kScanThreads / 64];
   const sfi::ScanSeg s = segs[blockIdx.y];
-  if (blockIdx.x * static_cast<size_t>(kScanPerBlock) >= s.words)
+  const size_t base = static_cast<size_t>(blockIdx.x) * kScanPerBlock;
+  if (base >= s.words)
     return;
   const size_t words = live_words(s);
-  const size_t base = static_cast<size_t>(blockIdx.x) * kScanPerBlock +
-                      static_cast<size_t>(threadIdx.x) * kPerThread;
-  uint32_t c[kPerThread];
-  uint32_t sum = 0;
-#pragma unroll
-  for (int i = 0; i < kPerThread; i++) {
-    c[i] = base + i < words ? static_cast<uint32_t>(__popc(s.bits[base + i])) : 0u;
-    sum += c[i];
+  const SF_GLOBAL uint32_t *bits = sfd::gmem(s.bits);
+  SF_GLOBAL uint32_t *prefix = sfd::gmem(s.prefix);
+  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
+  const size_t share = base + static_cast<size_t>(wave) * kWaveWords;
+  uint32_t mine = 0;
+  for (uint32_t i = 0; i < kWaveWords; i += 64) {
+    const size_t w = share + i + lane;
+    if (w < words)
+      mine += static_cast<uint32_t>(__popc(bits[w]));
   }
-  uint32_t total;
-  uint32_t run = block_inclusive(sum, &total) - sum + s.totals[blockIdx.x];
 #pragma unroll
-  for (int i = 0; i < kPerThread; i++) {
-    if (base + i < words)
-      s.prefix[base + i] = run;
-    run += c[i];
+  for (int d = 32; d >= 1; d >>= 1)
+    mine += __shfl_xor(mine, d);
+  if (lane == 0)
+    wave_totals[wave] = mine;
+  __syncthreads();
+  uint32_t run = s.totals[blockIdx.x];
+  for (uint32_t w = 0; w < wave; w++)
+    run += wave_totals[w];
+  for (uint32_t i = 0; i < kWaveWords; i += 64) {
+    const size_t w = share + i + lane;
+    const uint32_t c = w < words ? static_cast<uint32_t>(__popc(bits[w])) : 0u;
+    uint32_t incl = c;
+#pragma unroll
+    for (int d = 1; d < 64; d <<= 1) {
+      const uint32_t u = __shfl_up(incl, d);
+      if (lane >= static_cast<uint32_t>(d))
+        incl += u;
+    }
+    if (w < words)
+      prefix[w] = run + incl - c;
+    run += __shfl(incl, 63);
   }
 }
 
