@@ -93,6 +93,7 @@ private:
     size_t b_fused = 0, b_frames = 0, b_segs = 0;
     uint32_t items = 0;
     size_t max_words = 0;
+    int lane_columns = 4;
   };
   void ensure_slots(size_t n);
   void grow(DeviceBuffer &b, size_t bytes);

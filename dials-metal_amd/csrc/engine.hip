@@ -247,17 +247,18 @@ Engine::Prepared Engine::prepare(const FrameIn *frames, const int *ids,
   FusedParams *fp = reinterpret_cast<FusedParams *>(host);
   FrameDev *fd = reinterpret_cast<FrameDev *>(host + p.b_fused);
   ScanSeg *sg = reinterpret_cast<ScanSeg *>(host + p.b_fused + p.b_frames);
+  p.lane_columns = t.lane_columns;
   uint64_t strip_rows = 0;
   for (size_t i = 0; i < n; i++)
     strip_rows += static_cast<uint64_t>(slot(ids[i]).height) *
-                  strips_of(slot(ids[i]).width);
+                  strips_of(slot(ids[i]).width, p.lane_columns);
   const uint32_t seg = segment_rows(strip_rows, t);
   for (size_t i = 0; i < n; i++) {
     FrameSlot &s = slot(ids[i]);
     s.frame = frames[i].frame;
     s.kernel = kernel;
     FusedPlan plan;
-    const uint32_t k = plan_fused(plan, s.height, s.width, seg);
+    const uint32_t k = plan_fused(plan, s.height, s.width, seg, p.lane_columns);
     fp[i] = to_params(args(ids[i], frames[i].frame), plan, p.items);
     p.items += k;
     fd[i] = dev(ids[i], -1);
@@ -289,7 +290,7 @@ void Engine::enqueue_kernels(const Prepared &p, const FrameIn *frames,
     launch_fused_batch(fp_d, nf, p.items, table_.as<float>(),
                        reinterpret_cast<const double *>(table_.as<char>() +
                                                         128 * sizeof(float)),
-                       queue_.as<uint32_t>(), stream_);
+                       queue_.as<uint32_t>(), p.lane_columns, stream_);
   } else {
     for (size_t i = 0; i < n; i++)
       launch_direct(args(ids[i], frames[i].frame), scr0_.as<uint8_t>(),
@@ -338,7 +339,8 @@ void Engine::threshold(const FrameIn *frames, const int *ids, size_t n,
   std::vector<uint64_t> key = {static_cast<uint64_t>(kernel), epoch_,
                                static_cast<uint64_t>(t.seg_rows),
                                static_cast<uint64_t>(t.target_waves),
-                               static_cast<uint64_t>(t.waves_per_block)};
+                               static_cast<uint64_t>(t.waves_per_block),
+                               static_cast<uint64_t>(t.lane_columns)};
   for (size_t i = 0; i < n; i++)
     key.push_back((static_cast<uint64_t>(frames[i].height) << 32) |
                   (static_cast<uint64_t>(frames[i].width) << 8) |

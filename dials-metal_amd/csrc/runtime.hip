@@ -56,13 +56,14 @@ bool g_tuning_read = false;
 // Measured on MI355X over 16-frame batches of 16M frames: four waves a block
 // (the most resident waves under the LDS each wave's ring takes), segments
 // of 128-256 rows; a lone frame gets 32-row segments to fill the device.
-sf_tuning g_tuning = {0, 8192, 4, 32};
+sf_tuning g_tuning = {0, 8192, 4, 32, 4};
 
 bool tuning_valid(const sf_tuning &t) {
   return t.seg_rows >= 0 && t.seg_rows <= (1 << 20) && t.target_waves >= 1 &&
          (t.waves_per_block == 1 || t.waves_per_block == 2 ||
           t.waves_per_block == 4) &&
-         t.frames_per_launch >= 1 && t.frames_per_launch <= 4096;
+         t.frames_per_launch >= 1 && t.frames_per_launch <= 4096 &&
+         (t.lane_columns == 4 || t.lane_columns == 8);
 }
 
 // SPOTFINDER_HIP_TUNE="key=value,...": an unknown key or a bad value warns
@@ -93,6 +94,8 @@ void read_tuning_env() {
         t.waves_per_block = value;
       else if (key == "frames_per_launch")
         t.frames_per_launch = value;
+      else if (key == "lane_columns")
+        t.lane_columns = value;
       else
         known = false;
     }
@@ -717,34 +720,41 @@ int sf_autotune(const sf_frame_desc *frames, size_t n, int label,
   //    (median of three).
   sf_tuning best = start;
   double best_ms = 0.0;
+  const int lane_columns[] = {4, 8};
   const int waves_per_block[] = {1, 2, 4};
   const int seg_rows[] = {64, 128, 192, 256, 0, 0, 0};
   const int target_waves[] = {start.target_waves, start.target_waves,
                               start.target_waves, start.target_waves, 4096,
                               8192, 16384};
   sf_batch_set_timing(b, 1);
-  for (int w : waves_per_block)
-    for (size_t k = 0; k < sizeof(seg_rows) / sizeof(seg_rows[0]); k++) {
-      sf_tuning t = start;
-      t.waves_per_block = w;
-      t.seg_rows = seg_rows[k];
-      t.target_waves = target_waves[k];
-      if ((rc = sf_set_tuning(&t)) != 0)
+  std::vector<sf_tuning> shapes;
+  for (int kq : lane_columns)
+    for (int w : waves_per_block)
+      for (size_t k = 0; k < sizeof(seg_rows) / sizeof(seg_rows[0]); k++) {
+        sf_tuning t = start;
+        t.lane_columns = kq;
+        t.waves_per_block = w;
+        t.seg_rows = seg_rows[k];
+        t.target_waves = target_waves[k];
+        shapes.push_back(t);
+      }
+  for (const sf_tuning &t : shapes) {
+    if ((rc = sf_set_tuning(&t)) != 0)
+      return fail(rc);
+    double ms[3];
+    for (double &m : ms) {
+      if ((rc = sf_batch_threshold(b, frames, shape_n, 0)) != 0 ||
+          (rc = sf_batch_sync(b, counts.data(), status.data())) != 0 ||
+          (rc = sf_batch_kernel_ms(b, &m, nullptr)) != 0)
         return fail(rc);
-      double ms[3];
-      for (double &m : ms) {
-        if ((rc = sf_batch_threshold(b, frames, shape_n, 0)) != 0 ||
-            (rc = sf_batch_sync(b, counts.data(), status.data())) != 0 ||
-            (rc = sf_batch_kernel_ms(b, &m, nullptr)) != 0)
-          return fail(rc);
-      }
-      std::sort(ms, ms + 3);
-      report->shapes_tried++;
-      if (best_ms == 0.0 || ms[1] < best_ms) {
-        best_ms = ms[1];
-        best = t;
-      }
     }
+    std::sort(ms, ms + 3);
+    report->shapes_tried++;
+    if (best_ms == 0.0 || ms[1] < best_ms) {
+      best_ms = ms[1];
+      best = t;
+    }
+  }
   sf_batch_set_timing(b, 0);
   report->fused_ms_per_frame = best_ms / static_cast<double>(shape_n);
 

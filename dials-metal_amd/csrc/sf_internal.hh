@@ -121,22 +121,24 @@ struct FusedParams {
 // A frame's cut into work items (strips x segments) under a tuning.
 struct FusedPlan {
   uint32_t strips = 0, segments = 0, seg_rows = 0;
+  int lane_columns = 4;
 };
-// Strips of a frame's width; rows per segment for a batch of `strip_rows`
-// (the sum over its frames of height x strips) under a tuning; the plan.
-uint32_t strips_of(uint32_t width);
+// Strips of a frame's width with `lane_columns` columns a lane; rows per
+// segment for a batch of `strip_rows` (the sum over its frames of height x
+// strips) under a tuning; the plan.
+uint32_t strips_of(uint32_t width, int lane_columns);
 uint32_t segment_rows(uint64_t strip_rows, const Tuning &t);
 uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
-                    uint32_t seg_rows);
+                    uint32_t seg_rows, int lane_columns);
 FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,
                       uint32_t item_base);
 // One persistent launch over every item of `nframes` frames (descriptors in
-// device memory), mixed sizes and pixel types; each frame's mask and
-// overflow word cleared beforehand, and `queue`, the device word from which
-// waves take items, zeroed.
+// device memory, planned with `lane_columns`), mixed sizes and pixel types;
+// each frame's mask and overflow word cleared beforehand, and `queue`, the
+// device word from which waves take items, zeroed.
 void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
                         uint32_t total_items, const float *tf, const double *td,
-                        uint32_t *queue, hipStream_t stream);
+                        uint32_t *queue, int lane_columns, hipStream_t stream);
 
 // Clears `n` 32-bit words with a kernel, which unlike a memset node stays
 // correct when the sequence is captured into a graph and replayed.

@@ -2,28 +2,28 @@
 // pass of each wave down a column strip of the frame, reading every pixel
 // from HBM once.
 //
-// Geometry.  A wave owns a strip of 256 columns, four consecutive columns per
-// lane, and slides down a segment of rows.  The three windows nest: the
-// output needs stage 2's 11 x 11 sums (halo 5), which need stage 1's 5 x 5
-// erode (halo 2 more), which needs stage 0's 7 x 7 sums (halo 3 more), so the
-// strip carries 12 columns of halo on each side (10 needed, rounded to a lane)
-// and writes the middle 232; rows run from 10 above the segment to 10 below.
-// Pixels outside the frame count as masked, which is exact (a masked pixel
-// adds nothing to a sum, is set in scr0 so it cannot break an erode, and stays
-// out of stage 2's background), the same argument as dext_fused.hh:16-24.
+// Geometry.  A wave owns a strip of 64 KQ columns, KQ consecutive columns per
+// lane (KQ is 4 or 8, a launch-shape choice), and slides down a segment of
+// rows.  The three windows nest: the output needs stage 2's 11 x 11 sums (halo
+// 5), which need stage 1's 5 x 5 erode (halo 2 more), which needs stage 0's
+// 7 x 7 sums (halo 3 more), so the strip carries 10 columns of halo on each
+// side and writes the middle 64 KQ - 20; rows run from 10 above the segment to
+// 10 below.  Pixels outside the frame count as masked, which is exact (a
+// masked pixel adds nothing to a sum, is set in scr0 so it cannot break an
+// erode, and stays out of stage 2's background), the same argument as
+// dext_fused.hh:16-24.
 //
 // Representation.  A masked pixel is held as the value 0 with its bit in a
-// register history of masked nibbles (one nibble a row, a bit a column), so
-// every window sum is a plain sum of values and a window's count is the
-// window's size less the masked bits in it.  The entering row is read from
-// HBM two rows ahead of use; the rows leaving the windows are read back from a
-// 16-row ring in LDS that holds each pixel's low 16 bits, with bits 16-17 in a
-// byte ring written and read only for the rare rows that have such a pixel (a
-// wave-uniform row mask says which).  Sixteen rows suffice because the row 16
-// above is read before the entering row takes its slot.  The ring starts at
-// zero, so the rows before the first read back as nothing and no step needs a
-// prologue test.  With a 48-entry candidate stage a wave holds 9984 bytes of
-// LDS, so that four blocks of four waves fit a CU's 160 KB.
+// register history of masked nibbles (one nibble a row, a bit a column, one
+// history per four columns), so every window sum is a plain sum of values and
+// a window's count is the window's size less the masked bits in it.  The
+// entering row is read from HBM two rows ahead of use; the rows leaving the
+// windows are read back from a 16-row ring in LDS that holds each pixel's low
+// 16 bits, with bits 16-17 in a second ring written and read only for the
+// rare rows that have such a pixel (a wave-uniform row mask says which).
+// Sixteen rows suffice because the row 16 above is read before the entering
+// row takes its slot.  The ring starts at zero, so the rows before the first
+// read back as nothing and no step needs a prologue test.
 //
 // Per row entering at the bottom, the wave
 //   * updates stage 0's vertical 7-row sums of values and of squares per
@@ -33,15 +33,16 @@
 //     three rows up.  Where no masked pixel lies in any of the wave's windows
 //     (a wave-uniform test) every count is 49; elsewhere the counts come from
 //     the masked history by popcount and travel in the top bits of the sums;
-//   * erodes with bit operations on nibbles of scr0 (5 columns by 5 rows),
-//     finishing scr1 five rows up;
+//   * erodes with bit operations on scr0 (5 columns by 5 rows), finishing
+//     scr1 five rows up, and records which pixels stage 2 leaves out (masked
+//     or in scr1) in an exclusion history;
 //   * updates stage 2's vertical 11-row sums of values outside scr1 (only
 //     unmasked scr1 pixels need excluding, the masked ones being zero, and a
-//     wave-uniform test skips even that for rows with none), and, on rows with
-//     a candidate ten rows up, forms the counts from the histories, sums 11
-//     columns across lanes and tests, emitting signal pixels to a per-wave LDS
-//     stage (one atomic per flush) and setting their bits in the strong-pixel
-//     bitmask.
+//     wave-uniform record of the rows holding any skips even that for the
+//     rest), and, on rows with a candidate ten rows up, forms the counts from
+//     the exclusion history, sums 11 columns across lanes and tests, emitting
+//     signal pixels to a per-wave LDS stage (one atomic per flush) and setting
+//     their bits in the strong-pixel bitmask.
 //
 // Arithmetic.  Sums of values are exact in 32 bits while every valid pixel is
 // below 2^18 (49 * (2^18-1) < 2^26 under a count at bit 26; 121 * (2^18-1) <
@@ -54,6 +55,9 @@
 // an integer below 2^53, so it is formed with a fused multiply-add that is
 // exact, and the right side keeps the reference's two roundings; the stage-0
 // root comes from a host-computed table.
+#include <type_traits>
+#include <utility>
+
 #include "sf_device.hh"
 #include "sf_internal.hh"
 
@@ -62,17 +66,19 @@ namespace {
 using namespace sfd;
 
 constexpr int kLanes = 64;
-constexpr int kQ = 4;                   // columns per lane
-constexpr int kStrip = kQ * kLanes;     // 256 columns per wave
-constexpr int kHalo = 12;               // columns of halo each side
-constexpr int kOut = kStrip - 2 * kHalo; // 232 output columns per strip
+constexpr int kHalo = 10;               // columns of halo each side: 3 + 2 + 5
 constexpr int kRing = 16;               // raw rows held: entering .. 15 above
 constexpr uint32_t kStage = 48;         // candidates staged per wave
 constexpr uint32_t kFast = 1u << 18;    // fast-path bound on valid pixels
-constexpr uint32_t kWide = 1u << 16;    // pixels at or above this use the byte ring
+constexpr uint32_t kWide = 1u << 16;    // pixels at or above this use the high ring
 constexpr int kTable = 64;              // root table entries used (m <= 49)
 constexpr uint32_t kCount0 = 1u << 26;  // count unit in stage 0's word
 constexpr uint32_t kCount2 = 1u << 25;  // count unit in stage 2's word
+
+// Output columns of one strip with KQ columns a lane.
+__host__ __device__ constexpr int out_columns(int kq) {
+  return kq * kLanes - 2 * kHalo;
+}
 
 using sfi::FusedParams;
 
@@ -104,58 +110,110 @@ __device__ __forceinline__ double from_right(double v) {
       (static_cast<uint64_t>(hi) << 32) | lo));
 }
 
-// A lane's four pixels as one load; dword alignment is all the hardware
-// asks of a multi-dword global load.
+// The value that lane (this lane + D) holds.
+template <int D, typename V> __device__ __forceinline__ V from_lane(V v) {
+  if constexpr (D < 0)
+    return from_lane<D + 1>(from_left(v));
+  else if constexpr (D > 0)
+    return from_lane<D - 1>(from_right(v));
+  else
+    return v;
+}
+
+// The lane, relative to this one, that holds column c0 + I.
+template <int KQ, int I> constexpr int lane_of() {
+  return I >= 0 ? I / KQ : -((-I + KQ - 1) / KQ);
+}
+
+// e[j] = the value of column c0 - R + j, for j < KQ + 2R.
+template <int KQ, int R, typename V, int... J>
+__device__ __forceinline__ void gather(const V (&v)[KQ], V (&e)[KQ + 2 * R],
+                                       std::integer_sequence<int, J...>) {
+  ((e[J] = from_lane<lane_of<KQ, J - R>()>(
+        v[(J - R) - lane_of<KQ, J - R>() * KQ])),
+   ...);
+}
+
+template <int N, typename V> __device__ __forceinline__ V tree_sum(const V *e) {
+  if constexpr (N == 1)
+    return e[0];
+  else
+    return tree_sum<N / 2>(e) + tree_sum<N - N / 2>(e + N / 2);
+}
+
+// Sums over the 2R + 1 columns centred on each of a lane's KQ, with the
+// neighbouring lanes' columns by lane shifts: the first as a tree, each next
+// one from the last by the column leaving and the one entering.  The values
+// are integers (exact in doubles below 2^53, or modular in 32 bits), so the
+// order of the additions does not change the sums.
+template <int R, int KQ, typename V>
+__device__ __forceinline__ void across(const V (&v)[KQ], V (&h)[KQ]) {
+  V e[KQ + 2 * R];
+  gather<KQ, R>(v, e, std::make_integer_sequence<int, KQ + 2 * R>{});
+  h[0] = tree_sum<2 * R + 1>(e);
+#pragma unroll
+  for (int q = 1; q < KQ; q++)
+    h[q] = (h[q - 1] - e[q - 1]) + e[q + 2 * R];
+}
+
+// A lane's KQ pixels as vector loads; dword alignment is all the hardware
+// asks of a multi-dword global load (and uint16 rows get none: unaligned
+// global access is on).
 typedef uint32_t u32x4 __attribute__((ext_vector_type(4), aligned(4)));
 typedef uint16_t u16x4 __attribute__((ext_vector_type(4), aligned(2)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8), aligned(2)));
 typedef uint32_t u32x4a __attribute__((ext_vector_type(4))); // 16-byte aligned
 
-// Loads this lane's four pixels of in-frame row y; columns outside the frame
+// Loads this lane's KQ pixels of in-frame row y; columns outside the frame
 // read as 0 (their masked bits are constant per lane).
-template <typename T>
+template <typename T, int KQ>
 __device__ __forceinline__ void load_row(const SF_GLOBAL T *__restrict__ frame,
                                          int width, int y, int c0,
-                                         uint32_t (&v)[kQ]) {
+                                         uint32_t (&v)[KQ]) {
   const SF_GLOBAL T *row = frame + static_cast<size_t>(y) * width;
-  if (c0 >= 0 && c0 + kQ <= width) {
+  if (c0 >= 0 && c0 + KQ <= width) {
     if constexpr (sizeof(T) == 4) {
-      const u32x4 x = *reinterpret_cast<const SF_GLOBAL u32x4 *>(row + c0);
-      v[0] = x[0], v[1] = x[1], v[2] = x[2], v[3] = x[3];
+#pragma unroll
+      for (int k = 0; k < KQ; k += 4) {
+        const u32x4 x = *reinterpret_cast<const SF_GLOBAL u32x4 *>(row + c0 + k);
+        v[k] = x[0], v[k + 1] = x[1], v[k + 2] = x[2], v[k + 3] = x[3];
+      }
+    } else if constexpr (KQ == 8) {
+      const u16x8 x = *reinterpret_cast<const SF_GLOBAL u16x8 *>(row + c0);
+#pragma unroll
+      for (int q = 0; q < 8; q++)
+        v[q] = x[q];
     } else {
-      const u16x4 x = *reinterpret_cast<const SF_GLOBAL u16x4 *>(row + c0);
-      v[0] = x[0], v[1] = x[1], v[2] = x[2], v[3] = x[3];
+#pragma unroll
+      for (int k = 0; k < KQ; k += 4) {
+        const u16x4 x = *reinterpret_cast<const SF_GLOBAL u16x4 *>(row + c0 + k);
+        v[k] = x[0], v[k + 1] = x[1], v[k + 2] = x[2], v[k + 3] = x[3];
+      }
     }
     return;
   }
 #pragma unroll
-  for (int q = 0; q < kQ; q++) {
+  for (int q = 0; q < KQ; q++) {
     const int c = c0 + q;
     v[q] = (c >= 0 && c < width) ? static_cast<uint32_t>(row[c]) : 0u;
   }
 }
 
-// The byte ring's form of a lane's four pixels (each below 2^18): bits 16-17
-// of each.
-__device__ __forceinline__ uint32_t pack_high(const uint32_t (&v)[kQ]) {
-  return (v[0] >> 16) | ((v[1] >> 16) << 2) | ((v[2] >> 16) << 4) |
-         ((v[3] >> 16) << 6);
-}
-__device__ __forceinline__ void add_high(uint32_t h, uint32_t (&v)[kQ]) {
+// The high ring's form of a lane's pixels (each below 2^18): bits 16-17 of
+// column q at bits 2q, 2q + 1.
+template <int KQ>
+__device__ __forceinline__ uint32_t pack_high(const uint32_t (&v)[KQ]) {
+  uint32_t h = 0;
 #pragma unroll
-  for (int q = 0; q < kQ; q++)
-    v[q] |= ((h >> (2 * q)) & 3u) << 16;
+  for (int q = 0; q < KQ; q++)
+    h |= (v[q] >> 16) << (2 * q);
+  return h;
 }
-
-// Sums over 7 columns centred on each of a lane's four, c-3 .. c+3, with the
-// neighbouring lanes' columns by lane shifts.
-template <typename V>
-__device__ __forceinline__ void across7(const V (&v)[kQ], V (&h)[kQ]) {
-  const V l1 = from_left(v[1]), l2 = from_left(v[2]), l3 = from_left(v[3]);
-  const V r0 = from_right(v[0]), r1 = from_right(v[1]), r2 = from_right(v[2]);
-  h[0] = ((l1 + l2) + (l3 + v[0])) + ((v[1] + v[2]) + v[3]);
-  h[1] = (h[0] - l1) + r0;
-  h[2] = (h[1] - l2) + r1;
-  h[3] = (h[2] - l3) + r2;
+template <int KQ>
+__device__ __forceinline__ void add_high(uint32_t h, uint32_t (&v)[KQ]) {
+#pragma unroll
+  for (int q = 0; q < KQ; q++)
+    v[q] |= ((h >> (2 * q)) & 3u) << 16;
 }
 
 // Stage 0's test (dext.cc:99-110) on exact window sums: `count` valid pixels
@@ -199,10 +257,12 @@ __device__ __forceinline__ bool stage0_test<uint16_t>(uint32_t count,
 
 // What one wave keeps in LDS: the rings of raw rows (the low ring column by
 // column, so that a lane's 16-bit reads fall in distinct banks) and the
-// staged stage-2 candidates.
-struct WaveLds {
-  uint16_t ring[kRing][kQ][kLanes];
-  uint8_t high[kRing][kLanes];
+// staged stage-2 candidates.  KQ = 4 takes 9984 bytes (four blocks of four
+// waves fit a CU's 160 KB), KQ = 8 takes 19200 (two blocks of four).
+template <int KQ> struct WaveLds {
+  using High = std::conditional_t<KQ == 4, uint8_t, uint16_t>;
+  uint16_t ring[kRing][KQ][kLanes];
+  High high[kRing][kLanes];
   uint4 stage[kStage];
 };
 
@@ -216,33 +276,36 @@ __host__ __device__ constexpr uint64_t column_bits(int k0, int n, int q) {
 
 // One wave's sweep of one strip segment of one frame: `item` numbers the
 // frame's segments strip-fastest.
-template <typename T>
-__device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
-                      WaveLds &w, const typename traits<T>::R *table) {
+template <typename T, int KQ>
+__device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
+                                      uint32_t lane, WaveLds<KQ> &w,
+                                      const typename traits<T>::R *table) {
   using R = typename traits<T>::R;
   using Wt = typename traits<T>::W;
+  constexpr int P = KQ / 4;          // nibble histories, one per 4 columns
+  constexpr int kOut = out_columns(KQ);
   const int height = static_cast<int>(a.height);
   const int width = static_cast<int>(a.width);
   const int strip = static_cast<int>(item % a.strips);
   const int seg = static_cast<int>(item / a.strips);
   const int x0 = strip * kOut;
-  const int c0 = x0 - kHalo + kQ * static_cast<int>(lane); // first column
+  const int c0 = x0 - kHalo + KQ * static_cast<int>(lane); // first column
   const int y0 = seg * static_cast<int>(a.seg_rows);
   const int y1 = min(height, y0 + static_cast<int>(a.seg_rows));
   const SF_GLOBAL T *frame = gmem(static_cast<const T *>(a.frame));
   // records move as 16-byte vectors
   SF_GLOBAL u32x4a *out = reinterpret_cast<SF_GLOBAL u32x4a *>(gmem(a.out));
   SF_GLOBAL uint32_t *mask = gmem(a.mask);
-  uint16_t(*ring)[kQ][kLanes] = w.ring;
-  uint8_t(*high)[kLanes] = w.high;
-  auto ring_row = [&](int s, uint32_t(&v)[kQ]) {
+  uint16_t(*ring)[KQ][kLanes] = w.ring;
+  auto *high = w.high;
+  auto ring_row = [&](int s, uint32_t(&v)[KQ]) {
 #pragma unroll
-    for (int q = 0; q < kQ; q++)
+    for (int q = 0; q < KQ; q++)
       v[q] = ring[s][q][lane];
   };
   const uint32_t masked = traits<T>::masked;
   // A row whose pixels are all below this holds no masked pixel and none
-  // that needs the byte ring or the exact path.
+  // that needs the high ring or the exact path.
   const uint32_t plain = sizeof(T) == 4 ? kWide : masked;
   const R root49 = table[49];
 
@@ -323,7 +386,7 @@ __device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
   // it outputs: inside the strip's middle and the frame.
   uint32_t outside = 0, in_out = 0;
 #pragma unroll
-  for (int q = 0; q < kQ; q++) {
+  for (int q = 0; q < KQ; q++) {
     const int c = c0 + q;
     if (c < 0 || c >= width)
       outside |= 1u << q;
@@ -334,62 +397,78 @@ __device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
   // The ring starts at zero: rows before the first add and remove nothing.
   for (int s = 0; s < kRing; s++)
 #pragma unroll
-    for (int q = 0; q < kQ; q++)
+    for (int q = 0; q < KQ; q++)
       ring[s][q][lane] = 0;
 
-  uint32_t v0w[kQ] = {0, 0, 0, 0};       // stage 0 vertical: sum
-  double v0s[kQ] = {0.0, 0.0, 0.0, 0.0}; // stage 0 vertical: sum of squares
-  uint32_t v2w[kQ] = {0, 0, 0, 0};       // stage 2 vertical: sum
-  // Masked nibbles, row y_in - k at nibble k; rows before the first count as
-  // masked (their ring values are zero).
-  uint64_t mhist = ~0ull;
-  // Unmasked scr1 nibbles, row (y_in - 5) - k at nibble k.
-  uint64_t x1hist = 0;
-  uint32_t hreg = 0;      // horizontal-erode nibbles, row y_in-3 - k at nibble k
-  uint32_t wide_rows = 0; // wave-uniform; row y_in - k at bit k once y_in is in
+  uint32_t v0w[KQ] = {}; // stage 0 vertical: sum
+  double v0s[KQ] = {};   // stage 0 vertical: sum of squares
+  uint32_t v2w[KQ] = {}; // stage 2 vertical: sum
+  // Per four columns (columns 4h .. 4h + 3 of the lane):
+  //   mh[h]  masked nibbles, row y_in - k at nibble k (k < 8); rows before
+  //          the first count as masked (their ring values are zero);
+  //   xh[h]  excluded from stage 2 (masked or in scr1) nibbles, row
+  //          (y_in - 5) - k at nibble k (k < 12), the rows before the first
+  //          excluded;
+  //   hr[h]  horizontal-erode nibbles, row (y_in - 3) - k at nibble k (k < 5).
+  uint32_t mh[P], hr[P];
+  uint64_t xh[P];
+#pragma unroll
+  for (int h = 0; h < P; h++) {
+    mh[h] = ~0u;
+    xh[h] = ~0ull;
+    hr[h] = 0;
+  }
+  // Wave-uniform row records, row y_in - k at bit k once y_in is in: rows
+  // with a pixel at or above 2^16; rows (y_in - 5) - k with an unmasked scr1
+  // pixel anywhere in the wave.
+  uint32_t wide_rows = 0, x1_rows = 0;
 
   const int steps = (y1 - y0) + 20;
   // Two rows in flight from HBM: the buffer a row is read from takes the load
   // of the row two below as soon as its values are in registers.
-  uint32_t ahead0[kQ] = {0, 0, 0, 0}, ahead1[kQ] = {0, 0, 0, 0};
+  uint32_t ahead0[KQ] = {}, ahead1[KQ] = {};
   if (y0 - 10 >= 0 && y0 - 10 < height)
-    load_row<T>(frame, width, y0 - 10, c0, ahead0);
+    load_row<T, KQ>(frame, width, y0 - 10, c0, ahead0);
   if (y0 - 9 >= 0 && y0 - 9 < height)
-    load_row<T>(frame, width, y0 - 9, c0, ahead1);
+    load_row<T, KQ>(frame, width, y0 - 9, c0, ahead1);
   int slot = 0; // ring slot of the entering row
-  auto row = [&](int t, uint32_t(&buf)[kQ]) {
+  auto row = [&](int t, uint32_t(&buf)[KQ]) {
     const int y_in = y0 - 10 + t;
 
     // ---- the entering row: values with masked as zero, masked bits ----
-    uint32_t val[kQ];
-    uint32_t mnib = outside;
+    uint32_t val[KQ];
+    uint32_t mbits = outside;
     bool wide = false;
     if (y_in < 0 || y_in >= height) {
 #pragma unroll
-      for (int q = 0; q < kQ; q++)
+      for (int q = 0; q < KQ; q++)
         val[q] = 0;
-      mnib = 0xFu;
+      mbits = (1u << KQ) - 1u;
     } else {
+      uint32_t top = 0;
 #pragma unroll
-      for (int q = 0; q < kQ; q++)
+      for (int q = 0; q < KQ; q++) {
         val[q] = buf[q];
-      const uint32_t top = max(max(val[0], val[1]), max(val[2], val[3]));
+        top = max(top, val[q]);
+      }
       if (__builtin_expect(__ballot(top >= plain) != 0, 0)) {
         // Masked pixels become zero; a valid pixel at or above kFast voids
         // this frame's fast sums (the caller re-runs it exactly) and is
         // clamped so that no sum spills into its count bits.
+        uint32_t any = 0;
 #pragma unroll
-        for (int q = 0; q < kQ; q++) {
+        for (int q = 0; q < KQ; q++) {
           const bool m = val[q] >= masked;
-          mnib |= (m ? 1u : 0u) << q;
+          mbits |= (m ? 1u : 0u) << q;
           big |= !m && val[q] >= kFast;
           val[q] = m ? 0u : min(val[q], kFast - 1u);
+          any |= val[q];
         }
-        wide = __ballot((val[0] | val[1] | val[2] | val[3]) >= kWide) != 0;
+        wide = __ballot(any >= kWide) != 0;
       }
     }
     if (t + 2 < steps && y_in + 2 >= 0 && y_in + 2 < height)
-      load_row<T>(frame, width, y_in + 2, c0, buf);
+      load_row<T, KQ>(frame, width, y_in + 2, c0, buf);
 
     // Ring rows leaving stage 0 (y_in - 7), entering stage 2 (y_in - 5) and
     // leaving it (y_in - 16, whose slot the entering row takes once it has
@@ -399,27 +478,29 @@ __device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
     const int s16 = slot;
     // (Read in the same block as their first use, so that the 16-bit loads'
     // zero extension folds into them.)
-    uint32_t out0[kQ], in2[kQ], out2[kQ];
+    uint32_t out0[KQ], in2[KQ], out2[KQ];
     ring_row(s7, out0);
     ring_row(s5, in2);
     ring_row(s16, out2);
 #pragma unroll
-    for (int q = 0; q < kQ; q++)
+    for (int q = 0; q < KQ; q++)
       ring[slot][q][lane] = static_cast<uint16_t>(val[q]);
-    mhist = (mhist << 4) | mnib;
+#pragma unroll
+    for (int h = 0; h < P; h++)
+      mh[h] = (mh[h] << 4) | ((mbits >> (4 * h)) & 0xFu);
 
     // ---- stage 0: vertical 7-row sums (rows y_in-6 .. y_in) ----
 #pragma unroll
-    for (int q = 0; q < kQ; q++) {
+    for (int q = 0; q < KQ; q++) {
       const uint32_t d = val[q] - out0[q];
       v0w[q] += d;
       v0s[q] = __fma_rn(static_cast<double>(static_cast<int32_t>(d)),
                         static_cast<double>(val[q] + out0[q]), v0s[q]);
     }
     // Stage 2's entering and leaving rows, as a change to its sums.
-    uint32_t d2[kQ];
+    uint32_t d2[KQ];
 #pragma unroll
-    for (int q = 0; q < kQ; q++)
+    for (int q = 0; q < KQ; q++)
       d2[q] = in2[q] - out2[q];
 
     // The ring rows above hold only the low 16 bits.  For the rare rows with
@@ -427,60 +508,62 @@ __device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
     // k - 1 of wide_rows), add each such pixel's high part H to what used
     // it: stage 0's sums lose H more and their squares (o + H)^2 - o^2 more;
     // stage 2's entering and leaving values gain H.
-    uint32_t in2h[kQ] = {0, 0, 0, 0}, out2h[kQ] = {0, 0, 0, 0};
     if (__builtin_expect((wide_rows & ((1u << 6) | (1u << 4) | (1u << 15))) != 0,
                          0)) {
       if ((wide_rows >> 6) & 1u) {
-        uint32_t h[kQ] = {0, 0, 0, 0};
-        add_high(high[s7][lane], h);
+        uint32_t h[KQ] = {};
+        add_high<KQ>(high[s7][lane], h);
 #pragma unroll
-        for (int q = 0; q < kQ; q++) {
+        for (int q = 0; q < KQ; q++) {
           v0w[q] -= h[q];
           v0s[q] -= static_cast<double>(h[q]) *
                     static_cast<double>(2u * out0[q] + h[q]);
         }
       }
       if ((wide_rows >> 4) & 1u)
-        add_high(high[s5][lane], in2h);
+        add_high<KQ>(high[s5][lane], in2);
       if ((wide_rows >> 15) & 1u)
-        add_high(high[s16][lane], out2h);
+        add_high<KQ>(high[s16][lane], out2);
 #pragma unroll
-      for (int q = 0; q < kQ; q++) {
-        in2[q] += in2h[q];
-        out2[q] += out2h[q];
+      for (int q = 0; q < KQ; q++)
         d2[q] = in2[q] - out2[q];
-      }
     }
     wide_rows = (wide_rows << 1) | (wide ? 1u : 0u);
     if (__builtin_expect(wide, 0))
-      high[slot][lane] = static_cast<uint8_t>(pack_high(val));
+      high[slot][lane] = static_cast<typename WaveLds<KQ>::High>(pack_high<KQ>(val));
 
     // ---- stage 0: 7 columns across lanes, test -> scr0 of row y_in-3 ----
-    uint32_t scr0 = static_cast<uint32_t>(mhist >> 12) & 0xFu; // masked, y_in-3
+    uint32_t scr0 = 0; // starts as the masked bits of row y_in-3
+    uint32_t m7any = 0;
+#pragma unroll
+    for (int h = 0; h < P; h++) {
+      scr0 |= ((mh[h] >> 12) & 0xFu) << (4 * h);
+      m7any |= mh[h] & 0x0FFFFFFFu;
+    }
     {
-      double hs[kQ];
-      across7(v0s, hs);
+      double hs[KQ];
+      across<3>(v0s, hs);
       // A masked pixel in rows y_in-6 .. y_in of any lane: counts vary and
       // travel in the top bits of the sums; otherwise every count is 49.
-      const uint32_t m7 = static_cast<uint32_t>(mhist) & 0x0FFFFFFFu;
-      if (__ballot(m7 != 0) == 0) {
-        uint32_t hw[kQ];
-        across7(v0w, hw);
+      if (__ballot(m7any != 0) == 0) {
+        uint32_t hw[KQ];
+        across<3>(v0w, hw);
 #pragma unroll
-        for (int q = 0; q < kQ; q++)
+        for (int q = 0; q < KQ; q++)
           if (stage0_test<T>(49u, hw[q], hs[q], root49))
             scr0 |= 1u << q;
       } else {
-        uint32_t word[kQ], hw[kQ];
+        uint32_t word[KQ], hw[KQ];
 #pragma unroll
-        for (int q = 0; q < kQ; q++)
+        for (int q = 0; q < KQ; q++)
           word[q] = v0w[q] +
                     ((7u - static_cast<uint32_t>(__popc(
-                               m7 & static_cast<uint32_t>(column_bits(0, 7, q)))))
+                               mh[q / 4] &
+                               static_cast<uint32_t>(column_bits(0, 7, q % 4)))))
                      << 26);
-        across7(word, hw);
+        across<3>(word, hw);
 #pragma unroll
-        for (int q = 0; q < kQ; q++) {
+        for (int q = 0; q < KQ; q++) {
           const uint32_t m = hw[q] >> 26;
           if (stage0_test<T>(m, hw[q] & (kCount0 - 1), hs[q], table[m]))
             scr0 |= 1u << q;
@@ -489,77 +572,87 @@ __device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
     }
 
     // ---- stage 1: erode 5 x 5 -> scr1 of row y_in-5 ----
-    uint32_t x1; // scr1 and not masked
+    uint32_t x1 = 0, ex = 0; // scr1 and not masked; scr1 or masked
     {
-      const uint32_t left = from_left(scr0) & 0xFu;
-      const uint32_t right = from_right(scr0) & 0xFu;
-      // bit i is column c0 - 4 + i
-      const uint32_t x = left | (scr0 << 4) | (right << 8);
+      const uint32_t left = from_left(scr0);
+      const uint32_t right = from_right(scr0);
+      // bit i is column c0 - 2 + i
+      const uint32_t x = ((left >> (KQ - 2)) & 3u) | (scr0 << 2) |
+                         ((right & 3u) << (KQ + 2));
+      // bit i: columns c0 - 2 + i .. c0 + 2 + i all set
       const uint32_t y = x & (x >> 1) & (x >> 2) & (x >> 3) & (x >> 4);
-      hreg = (hreg << 4) | ((y >> 2) & 0xFu);
-      const uint32_t all5 =
-          hreg & (hreg >> 4) & (hreg >> 8) & (hreg >> 12) & (hreg >> 16) & 0xFu;
-      // scr1 = all5 | masked; a masked pixel adds zero to every sum, so only
-      // the unmasked part of scr1 needs excluding.
-      x1 = all5 & ~(static_cast<uint32_t>(mhist >> 20) & 0xFu);
+#pragma unroll
+      for (int h = 0; h < P; h++) {
+        hr[h] = (hr[h] << 4) | ((y >> (4 * h)) & 0xFu);
+        const uint32_t all5 = hr[h] & (hr[h] >> 4) & (hr[h] >> 8) &
+                              (hr[h] >> 12) & (hr[h] >> 16) & 0xFu;
+        // scr1 = all5 | masked; a masked pixel adds zero to every sum, so
+        // only the unmasked part of scr1 needs excluding from the values.
+        const uint32_t m5 = (mh[h] >> 20) & 0xFu;
+        x1 |= (all5 & ~m5) << (4 * h);
+        ex |= (all5 | m5) << (4 * h);
+      }
     }
+    x1_rows = (x1_rows << 1) | (__ballot(x1 != 0) != 0 ? 1u : 0u);
 
     // ---- stage 2: vertical 11-row sums of values outside scr1 ----
     {
-      const uint32_t xout = static_cast<uint32_t>(x1hist >> 40) & 0xFu; // y_in-16
-      if (__ballot((x1 | xout) != 0) == 0) {
+      if ((x1_rows & ((1u << 0) | (1u << 11))) == 0) {
+        // no unmasked scr1 pixel in the wave in rows y_in-5 and y_in-16
 #pragma unroll
-        for (int q = 0; q < kQ; q++)
+        for (int q = 0; q < KQ; q++)
           v2w[q] += d2[q];
       } else {
+        // Row y_in-16's exclusion bits; its masked pixels are zero anyway.
+        uint32_t xout = 0;
 #pragma unroll
-        for (int q = 0; q < kQ; q++)
+        for (int h = 0; h < P; h++)
+          xout |= (static_cast<uint32_t>(xh[h] >> 40) & 0xFu) << (4 * h);
+#pragma unroll
+        for (int q = 0; q < KQ; q++)
           v2w[q] += (((x1 >> q) & 1u) ? 0u : in2[q]) -
                     (((xout >> q) & 1u) ? 0u : out2[q]);
       }
-      x1hist = (x1hist << 4) | x1;
+#pragma unroll
+      for (int h = 0; h < P; h++)
+        xh[h] = (xh[h] << 4) | ((ex >> (4 * h)) & 0xFu);
     }
 
     // ---- stage 2: candidates of row y_in-10 ----
-    if (t >= 20) {
+    if (t >= 20 && ((x1_rows >> 5) & 1u)) {
       const int y2 = y_in - 10;
-      const uint32_t maybe = static_cast<uint32_t>(x1hist >> 20) & in_out;
-      if (__ballot(maybe != 0) != 0) {
-        // Counts: rows y_in-15 .. y_in-5, less masked and unmasked-scr1.
-        uint32_t word[kQ];
+      // Excluded pixels of row y_in-10 in the output columns; of them the
+      // candidates are the unmasked ones with a value above zero, and a
+      // masked pixel's ring value is zero.
+      uint32_t maybe = 0;
 #pragma unroll
-        for (int q = 0; q < kQ; q++) {
-          const uint32_t excluded =
-              static_cast<uint32_t>(__popcll(mhist & column_bits(5, 11, q))) +
-              static_cast<uint32_t>(__popcll(x1hist & column_bits(0, 11, q)));
-          word[q] = v2w[q] + ((11u - excluded) << 25);
-        }
-        const uint32_t l0 = from_left(word[0]), l1 = from_left(word[1]),
-                       l2 = from_left(word[2]), l3 = from_left(word[3]);
-        const uint32_t r0 = from_right(word[0]), r1 = from_right(word[1]),
-                       r2 = from_right(word[2]), r3 = from_right(word[3]);
-        const uint32_t ll3 = from_left(l3), rr0 = from_right(r0);
-        uint32_t h[kQ];
-        h[0] = ll3 + l0 + l1 + l2 + l3 + word[0] + word[1] + word[2] + word[3] +
-               r0 + r1;
-        h[1] = h[0] - ll3 + r2;
-        h[2] = h[1] - l0 + r3;
-        h[3] = h[2] - l1 + rr0;
-        const int s10 = (slot - 10) & (kRing - 1);
-        uint32_t raw[kQ];
-        ring_row(s10, raw);
-        if ((wide_rows >> 10) & 1u)
-          add_high(high[s10][lane], raw);
+      for (int h = 0; h < P; h++)
+        maybe |= (static_cast<uint32_t>(xh[h] >> 20) & 0xFu) << (4 * h);
+      maybe &= in_out;
+      // Counts: rows y_in-15 .. y_in-5, less the excluded.
+      uint32_t word[KQ];
 #pragma unroll
-        for (int q = 0; q < kQ; q++) {
-          const int c = c0 + q;
-          const bool cand = ((maybe >> q) & 1u) && raw[q] > 0;
-          const uint4 entry = make_uint4(
-              static_cast<uint32_t>(y2) * a.width + static_cast<uint32_t>(c),
-              raw[q] | (static_cast<uint32_t>(c & 31) << 18), h[q],
-              static_cast<uint32_t>(y2) * a.wpr + static_cast<uint32_t>(c) / 32);
-          put(cand, entry);
-        }
+      for (int q = 0; q < KQ; q++) {
+        const uint32_t excluded = static_cast<uint32_t>(
+            __popcll(xh[q / 4] & column_bits(0, 11, q % 4)));
+        word[q] = v2w[q] + ((11u - excluded) << 25);
+      }
+      uint32_t hsum[KQ];
+      across<5>(word, hsum);
+      const int s10 = (slot - 10) & (kRing - 1);
+      uint32_t raw[KQ];
+      ring_row(s10, raw);
+      if ((wide_rows >> 10) & 1u)
+        add_high<KQ>(high[s10][lane], raw);
+#pragma unroll
+      for (int q = 0; q < KQ; q++) {
+        const int c = c0 + q;
+        const bool cand = ((maybe >> q) & 1u) && raw[q] > 0;
+        const uint4 entry = make_uint4(
+            static_cast<uint32_t>(y2) * a.width + static_cast<uint32_t>(c),
+            raw[q] | (static_cast<uint32_t>(c & 31) << 18), hsum[q],
+            static_cast<uint32_t>(y2) * a.wpr + static_cast<uint32_t>(c) / 32);
+        put(cand, entry);
       }
     }
     slot = (slot + 1) & (kRing - 1);
@@ -582,12 +675,12 @@ __device__ void sweep(const FusedParams &a, uint32_t item, uint32_t lane,
 // types in one launch: items of unequal cost then finish within about one
 // item of each other.  The grid is the device's resident waves, so the launch
 // is persistent, and every wave leaves when the queue runs past the items.
-template <int Waves>
+template <int Waves, int KQ>
 __global__ __launch_bounds__(Waves *kLanes) void fused_kernel(
     const FusedParams *__restrict__ frames, uint32_t nframes,
     uint32_t total_items, const float *__restrict__ tf,
     const double *__restrict__ td, uint32_t *__restrict__ queue) {
-  __shared__ WaveLds lds[Waves];
+  __shared__ WaveLds<KQ> lds[Waves];
   __shared__ float table_f[kTable];
   __shared__ double table_d[kTable];
   for (int i = threadIdx.x; i < kTable; i += blockDim.x) {
@@ -616,41 +709,61 @@ __global__ __launch_bounds__(Waves *kLanes) void fused_kernel(
     lo = __builtin_amdgcn_readfirstlane(lo);
     const FusedParams p = frames[lo];
     if (p.bits == 16)
-      sweep<uint16_t>(p, item - p.item_base, lane, lds[wave], table_f);
+      sweep<uint16_t, KQ>(p, item - p.item_base, lane, lds[wave], table_f);
     else
-      sweep<uint32_t>(p, item - p.item_base, lane, lds[wave], table_d);
+      sweep<uint32_t, KQ>(p, item - p.item_base, lane, lds[wave], table_d);
   }
 }
 
 // Resident blocks of a kernel variant on one CU, from the occupancy the
 // compiler's register and LDS use allow; computed once.
-template <int Waves> uint32_t resident_blocks() {
+template <int Waves, int KQ> uint32_t resident_blocks() {
   static const uint32_t blocks = [] {
     int n = 0;
     sfi::check(hipOccupancyMaxActiveBlocksPerMultiprocessor(
-                   &n, fused_kernel<Waves>, Waves * kLanes, 0),
+                   &n, fused_kernel<Waves, KQ>, Waves * kLanes, 0),
                "querying the fused kernel's occupancy");
     return static_cast<uint32_t>(n > 0 ? n : 1);
   }();
   return blocks;
 }
 
-template <int Waves>
+template <int Waves, int KQ>
 void launch_waves(const FusedParams *frames, uint32_t nframes,
                   uint32_t total_items, const float *tf, const double *td,
                   uint32_t *queue, uint32_t cus, hipStream_t stream) {
-  const uint32_t resident = cus * resident_blocks<Waves>();
+  const uint32_t resident = cus * resident_blocks<Waves, KQ>();
   const uint32_t need = (total_items + Waves - 1) / Waves;
   const uint32_t blocks = need < resident ? need : resident;
-  fused_kernel<Waves><<<blocks, Waves * kLanes, 0, stream>>>(
+  fused_kernel<Waves, KQ><<<blocks, Waves * kLanes, 0, stream>>>(
       frames, nframes, total_items, tf, td, queue);
+}
+
+template <int KQ>
+void launch_shape(int waves, const FusedParams *frames, uint32_t nframes,
+                  uint32_t total_items, const float *tf, const double *td,
+                  uint32_t *queue, uint32_t cus, hipStream_t stream) {
+  switch (waves) {
+  case 1:
+    launch_waves<1, KQ>(frames, nframes, total_items, tf, td, queue, cus, stream);
+    break;
+  case 4:
+    launch_waves<4, KQ>(frames, nframes, total_items, tf, td, queue, cus, stream);
+    break;
+  default:
+    launch_waves<2, KQ>(frames, nframes, total_items, tf, td, queue, cus, stream);
+    break;
+  }
 }
 
 } // namespace
 
 namespace sfi {
 
-uint32_t strips_of(uint32_t width) { return (width + kOut - 1) / kOut; }
+uint32_t strips_of(uint32_t width, int lane_columns) {
+  const uint32_t out = static_cast<uint32_t>(out_columns(lane_columns));
+  return (width + out - 1) / out;
+}
 
 uint32_t segment_rows(uint64_t strip_rows, const Tuning &t) {
   if (t.seg_rows > 0)
@@ -665,16 +778,17 @@ uint32_t segment_rows(uint64_t strip_rows, const Tuning &t) {
 }
 
 uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
-                    uint32_t seg_rows) {
-  p.strips = strips_of(width);
+                    uint32_t seg_rows, int lane_columns) {
+  p.strips = strips_of(width, lane_columns);
   p.seg_rows = seg_rows;
   p.segments = (height + seg_rows - 1) / seg_rows;
+  p.lane_columns = lane_columns;
   return p.strips * p.segments;
 }
 
 void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
                         uint32_t total_items, const float *tf, const double *td,
-                        uint32_t *queue, hipStream_t stream) {
+                        uint32_t *queue, int lane_columns, hipStream_t stream) {
   if (total_items == 0)
     return;
   const Tuning t = tuning();
@@ -684,17 +798,12 @@ void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
     (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount,
                                 device);
   const uint32_t n = static_cast<uint32_t>(cus);
-  switch (t.waves_per_block) {
-  case 1:
-    launch_waves<1>(frames_device, nframes, total_items, tf, td, queue, n, stream);
-    break;
-  case 4:
-    launch_waves<4>(frames_device, nframes, total_items, tf, td, queue, n, stream);
-    break;
-  default:
-    launch_waves<2>(frames_device, nframes, total_items, tf, td, queue, n, stream);
-    break;
-  }
+  if (lane_columns == 8)
+    launch_shape<8>(t.waves_per_block, frames_device, nframes, total_items, tf,
+                    td, queue, n, stream);
+  else
+    launch_shape<4>(t.waves_per_block, frames_device, nframes, total_items, tf,
+                    td, queue, n, stream);
   check(hipGetLastError(), "launching the fused threshold");
 }
 

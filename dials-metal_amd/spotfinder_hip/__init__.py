@@ -58,7 +58,8 @@ class FrameDesc(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("seg_rows", ctypes.c_int), ("target_waves", ctypes.c_int),
                 ("waves_per_block", ctypes.c_int),
-                ("frames_per_launch", ctypes.c_int)]
+                ("frames_per_launch", ctypes.c_int),
+                ("lane_columns", ctypes.c_int)]
 
     def as_dict(self):
         return {f: getattr(self, f) for f, _ in self._fields_}

@@ -116,18 +116,20 @@ void sf_set_kernel(int kernel);
 int sf_get_kernel(void);
 
 /* The fused kernel's launch shape, which the autotuner writes.  A wave sweeps
- * a 256-column strip down `seg_rows` rows (plus 20 rows of halo); seg_rows 0
- * sizes segments so that one frame makes about `target_waves` waves.
- * `waves_per_block` is 1, 2 or 4; `frames_per_launch` is the batched mode's
- * batch.  No counterpart in the reference, whose launch shape is fixed
- * (dext_cuda.cu:728-730).  From SPOTFINDER_HIP_TUNE, e.g.
- * "seg_rows=128,target_waves=4096,waves_per_block=2,frames_per_launch=32",
- * when set.  sf_set_tuning returns -1 for a value out of range. */
+ * a strip of 64 x `lane_columns` columns (4 or 8 a lane) down `seg_rows` rows
+ * (plus 20 rows of halo); seg_rows 0 sizes segments so that one frame makes
+ * about `target_waves` waves.  `waves_per_block` is 1, 2 or 4;
+ * `frames_per_launch` is the batched mode's batch.  No counterpart in the
+ * reference, whose launch shape is fixed (dext_cuda.cu:728-730).  From
+ * SPOTFINDER_HIP_TUNE, e.g. "seg_rows=128,target_waves=4096,
+ * waves_per_block=2,frames_per_launch=32,lane_columns=8", when set.
+ * sf_set_tuning returns -1 for a value out of range. */
 typedef struct sf_tuning {
   int seg_rows;
   int target_waves;
   int waves_per_block;
   int frames_per_launch;
+  int lane_columns;
 } sf_tuning;
 void sf_get_tuning(sf_tuning *t);
 int sf_set_tuning(const sf_tuning *t);
