@@ -202,14 +202,14 @@ FrameDev Engine::dev(int id, int prev) {
 ScanSeg Engine::mask_seg(int id) {
   FrameSlot &s = slot(id);
   return ScanSeg{s.mask.as<uint32_t>(), s.prefix.as<uint32_t>(),
-                 s.totals.as<uint32_t>(), s.words, nullptr, 0};
+                 s.totals.as<uint32_t>(), s.words, nullptr, 0, 1};
 }
 
 ScanSeg Engine::root_seg(int id) {
   FrameSlot &s = slot(id);
   return ScanSeg{s.roots.as<uint32_t>(), s.root_prefix.as<uint32_t>(),
                  s.root_totals.as<uint32_t>(), s.root_words,
-                 counters_.as<uint32_t>() + kCounters * id, s.capacity};
+                 counters_.as<uint32_t>() + kCounters * id, s.capacity, 0};
 }
 
 FrameArgs Engine::args(int id, const void *frame) {

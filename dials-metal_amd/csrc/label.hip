@@ -21,10 +21,11 @@
 //   * roots are numbered by a popcount scan of a root bitmask, which numbers
 //     components in order of first appearance -- the reference's order;
 //   * the runs are sorted by component (one atomic per run takes its place,
-//     a scan of the counts gives each component's first), and one thread per
-//     component sums its runs' pixels' exact integer moments in registers
-//     (128-bit where a product can exceed 64 bits) and writes the record whole
-//     -- no atomics on the records -- and its pixels' component.
+//     a scan of the counts gives each component's first), and a group of
+//     eight lanes per component sums its runs' pixels' exact integer moments
+//     in registers (128-bit where a product can exceed 64 bits), adds the
+//     lanes' sums by lane shuffles and writes the record whole -- no atomics
+//     on the records -- and its pixels' component.
 // Links to the previous frame (pixels at the same position in both frames)
 // are listed per run, as the components of the previous frame's bitmask
 // segments under the run, for the host's three-dimensional stitch
@@ -121,33 +122,70 @@ __device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t lane,
   return incl - v;
 }
 
-// Runs starting in this lane's words of bitmask row `row` (words lane,
-// lane + 64, ...): a run starts at a set bit whose left neighbour in the row
-// is clear.
-__device__ __forceinline__ uint32_t lane_runs(const SF_GLOBAL uint32_t *row,
-                                              uint32_t wpr, uint32_t lane) {
-  uint32_t count = 0;
-  for (uint32_t j = lane; j < wpr; j += kLanes) {
-    const uint32_t m = row[j];
-    if (m == 0)
-      continue;
-    const uint32_t carry = j > 0 ? row[j - 1] >> 31 : 0u;
-    count += static_cast<uint32_t>(__popc(m & ~((m << 1) | carry)));
-  }
-  return count;
+// Bitmask words a lane holds at once in count_runs and emit_runs: a row of
+// up to 64 x kRowChunks words (16384 columns) in one round of loads.
+constexpr uint32_t kRowChunks = 4;
+
+// Lane shifts across the whole wave (DPP wave_shr:1 / wave_shl:1 with
+// bound_ctrl): lane i gets lane i-1's value (lane 0 gets 0), or lane i+1's
+// (lane 63 gets 0).
+__device__ __forceinline__ uint32_t from_left(uint32_t v) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x138, 0xf, 0xf, true));
+}
+__device__ __forceinline__ uint32_t from_right(uint32_t v) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x130, 0xf, 0xf, true));
+}
+__device__ __forceinline__ uint32_t lane_value(uint32_t v, uint32_t l) {
+  return static_cast<uint32_t>(
+      __builtin_amdgcn_readlane(static_cast<int>(v), static_cast<int>(l)));
 }
 
-// Runs per bitmask row of every frame, one wave per row.
+// Run starts in a round of a bitmask row held one word a lane per chunk
+// (lane l, chunk c: word j0 + 64 c + l): a run starts at a set bit whose left
+// neighbour in the row is clear.  `carry_in` is bit 31 of the word before
+// the round's first, `left_of` gives it for the next round.
+struct RowRound {
+  uint32_t m[kRowChunks];
+  __device__ __forceinline__ uint32_t starts(uint32_t c, uint32_t lane,
+                                             uint32_t carry_in) const {
+    uint32_t left = from_left(m[c]);
+    const uint32_t first_left = c == 0 ? carry_in : lane_value(m[c - 1], kLanes - 1);
+    if (lane == 0)
+      left = first_left;
+    return m[c] & ~((m[c] << 1) | (left >> 31));
+  }
+  __device__ __forceinline__ uint32_t left_of() const {
+    return lane_value(m[kRowChunks - 1], kLanes - 1);
+  }
+};
+
+// Runs per bitmask row of every frame, one wave per row, the row's words
+// loaded in one round.
 __global__ __launch_bounds__(256) void count_runs(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   if (live_count(f) == 0)
     return;
   const uint32_t lane = threadIdx.x % kLanes;
   const uint32_t per_block = blockDim.x / kLanes;
+  const uint32_t wpr = f.wpr;
   for (uint32_t y = blockIdx.x * per_block + threadIdx.x / kLanes; y < f.height;
        y += gridDim.x * per_block) {
-    const uint32_t count =
-        lane_runs(gmem(f.mask) + static_cast<size_t>(y) * f.wpr, f.wpr, lane);
+    const SF_GLOBAL uint32_t *row = gmem(f.mask) + static_cast<size_t>(y) * wpr;
+    uint32_t count = 0, carry_in = 0;
+    for (uint32_t j0 = 0; j0 < wpr; j0 += kLanes * kRowChunks) {
+      RowRound r;
+#pragma unroll
+      for (uint32_t c = 0; c < kRowChunks; c++) {
+        const uint32_t j = j0 + c * kLanes + lane;
+        r.m[c] = j < wpr ? row[j] : 0u;
+      }
+#pragma unroll
+      for (uint32_t c = 0; c < kRowChunks; c++)
+        count += static_cast<uint32_t>(__popc(r.starts(c, lane, carry_in)));
+      carry_in = r.left_of();
+    }
     uint32_t total;
     (void)wave_prefix(count, lane, &total);
     if (lane == 0)
@@ -196,50 +234,83 @@ __global__ __launch_bounds__(1024) void scan_runs(const FrameDev *__restrict__ f
 
 // The runs of every frame, one wave per bitmask row, each row's at its place
 // in the frame's run list; every pixel of a run points at the run's first
-// rank (the union-find's initial forest).
+// rank (the union-find's initial forest).  A row's words and rank prefixes
+// are loaded in one round (lane l holds words l, l + 64, ...), neighbouring
+// words come from the neighbouring lanes, and only a run that crosses a whole
+// word reads the row again.  Rows without runs are skipped from the row
+// scan alone.
 __global__ __launch_bounds__(256) void emit_runs(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   if (live_count(f) == 0)
     return;
   const uint32_t lane = threadIdx.x % kLanes;
   const uint32_t per_block = blockDim.x / kLanes;
+  const uint32_t wpr = f.wpr;
+  const uint32_t nr = f.counter[4];
+  const SF_GLOBAL uint32_t *bases = gmem(f.row_runs);
   SF_GLOBAL uint32_t *parent = gmem(f.parent);
   SF_GLOBAL run4 *runs = run_list(f);
   for (uint32_t y = blockIdx.x * per_block + threadIdx.x / kLanes; y < f.height;
        y += gridDim.x * per_block) {
-    const SF_GLOBAL uint32_t *row = gmem(f.mask) + static_cast<size_t>(y) * f.wpr;
-    const SF_GLOBAL uint32_t *pre = gmem(f.prefix) + static_cast<size_t>(y) * f.wpr;
-    uint32_t total;
-    uint32_t k = wave_prefix(lane_runs(row, f.wpr, lane), lane, &total);
-    if (total == 0)
+    uint32_t k = bases[y];
+    const uint32_t next = y + 1 < f.height ? bases[y + 1] : nr;
+    if (next == k)
       continue;
-    k += gmem(f.row_runs)[y];
-    for (uint32_t j = lane; j < f.wpr; j += kLanes) {
-      const uint32_t m = row[j];
-      if (m == 0)
-        continue;
-      const uint32_t carry = j > 0 ? row[j - 1] >> 31 : 0u;
-      uint32_t starts = m & ~((m << 1) | carry);
-      while (starts) {
-        const uint32_t b = static_cast<uint32_t>(__ffs(starts)) - 1u;
-        starts &= starts - 1u;
-        const uint32_t r0 = pre[j] + static_cast<uint32_t>(__popc(m & ((1u << b) - 1u)));
-        // The run's length: set bits from b on, into the row's next words.
-        uint32_t len = 0, jj = j, w = m >> b, avail = 32u - b;
-        for (;;) {
-          const uint32_t ones =
-              ~w == 0u ? 32u : static_cast<uint32_t>(__ffs(~w)) - 1u;
-          const uint32_t t = ones < avail ? ones : avail;
-          len += t;
-          if (t < avail || ++jj >= f.wpr)
-            break;
-          w = row[jj];
-          avail = 32u;
-        }
-        runs[k++] = run4{r0, 32u * j + b, y, len};
-        for (uint32_t r = r0; r < r0 + len; r++)
-          parent[r] = r0;
+    const SF_GLOBAL uint32_t *row = gmem(f.mask) + static_cast<size_t>(y) * wpr;
+    const SF_GLOBAL uint32_t *pre = gmem(f.prefix) + static_cast<size_t>(y) * wpr;
+    uint32_t carry_in = 0; // bit 31 of the word before this round's first
+    for (uint32_t j0 = 0; j0 < wpr; j0 += kLanes * kRowChunks) {
+      RowRound rr;
+      uint32_t p[kRowChunks];
+#pragma unroll
+      for (uint32_t c = 0; c < kRowChunks; c++) {
+        const uint32_t j = j0 + c * kLanes + lane;
+        rr.m[c] = j < wpr ? row[j] : 0u;
+        p[c] = j < wpr ? pre[j] : 0u;
       }
+      const uint32_t(&m)[kRowChunks] = rr.m;
+#pragma unroll
+      for (uint32_t c = 0; c < kRowChunks; c++) {
+        const uint32_t j = j0 + c * kLanes + lane;
+        uint32_t right = from_right(m[c]);
+        if (c + 1 < kRowChunks) {
+          const uint32_t next_first = lane_value(m[c + 1], 0);
+          if (lane == kLanes - 1)
+            right = next_first;
+        }
+        uint32_t starts = rr.starts(c, lane, carry_in);
+        uint32_t total;
+        uint32_t at = k + wave_prefix(static_cast<uint32_t>(__popc(starts)), lane, &total);
+        while (starts) {
+          const uint32_t b = static_cast<uint32_t>(__ffs(starts)) - 1u;
+          starts &= starts - 1u;
+          const uint32_t r0 = p[c] + static_cast<uint32_t>(__popc(m[c] & ((1u << b) - 1u)));
+          // The run's length: set bits from b on, into the row's next words.
+          const uint32_t w = ~(m[c] >> b);
+          uint32_t len = w == 0u ? 32u : static_cast<uint32_t>(__ffs(w)) - 1u;
+          if (len >= 32u - b) {
+            len = 32u - b;
+            // The next word is the right neighbour's, except past this
+            // round's last chunk, where it is read again.
+            uint32_t jj = j + 1;
+            uint32_t nw = (c + 1 < kRowChunks || lane + 1 < kLanes)
+                              ? right
+                              : (jj < wpr ? row[jj] : 0u);
+            for (;;) {
+              const uint32_t t = ~nw == 0u ? 32u : static_cast<uint32_t>(__ffs(~nw)) - 1u;
+              len += t;
+              if (t < 32u || ++jj >= wpr)
+                break;
+              nw = row[jj];
+            }
+          }
+          runs[at++] = run4{r0, 32u * j + b, y, len};
+          for (uint32_t r = r0; r < r0 + len; r++)
+            parent[r] = r0;
+        }
+        k += total;
+      }
+      carry_in = rr.left_of();
     }
   }
 }
@@ -409,10 +480,31 @@ __device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t l,
   lo = s;
 }
 
-// Exact moments, one thread per component: its runs (in component order) and
-// their pixels (consecutive ranks of the ordered list) summed in registers,
-// the record written whole -- no atomics -- and every pixel's component
-// written.
+// Lanes that sum one component together: a component has up to a few tens
+// of runs of a few pixels each.
+constexpr uint32_t kGroup = 8;
+
+// Totals over the kGroup lanes of a group (lanes g .. g + kGroup - 1, g a
+// multiple of kGroup), every lane of the group getting them.
+__device__ __forceinline__ void group_add(uint64_t &lo, uint64_t &hi) {
+#pragma unroll
+  for (uint32_t d = 1; d < kGroup; d <<= 1)
+    acc128(lo, hi, __shfl_xor(lo, static_cast<int>(d)),
+           __shfl_xor(hi, static_cast<int>(d)));
+}
+template <typename V, typename Op>
+__device__ __forceinline__ V group_reduce(V v, Op op) {
+#pragma unroll
+  for (uint32_t d = 1; d < kGroup; d <<= 1)
+    v = op(v, static_cast<V>(__shfl_xor(v, static_cast<int>(d))));
+  return v;
+}
+
+// Exact moments, a group of kGroup lanes per component: each lane sums every
+// kGroup-th of the component's runs (in component order) and their pixels
+// (consecutive ranks of the ordered list) in registers, the group adds the
+// lanes' sums, and its first lane writes the record whole -- no atomics --
+// while every lane writes its pixels' component.
 __global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   const uint32_t nc = f.counter[2];
@@ -421,42 +513,81 @@ __global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ f
   const SF_GLOBAL run4 *runs = run_list(f);
   const SF_GLOBAL sf_signal_pixel *ordered = gmem(f.ordered);
   SF_GLOBAL uint32_t *comp_of = gmem(f.comp_of);
-  for (uint32_t c = blockIdx.x * blockDim.x + threadIdx.x; c < nc;
-       c += gridDim.x * blockDim.x) {
-    sf_component z{};
-    z.first = z.x0 = z.y0 = 0xFFFFFFFFu;
+  const uint32_t sub = threadIdx.x % kGroup;
+  const uint32_t groups = gridDim.x * blockDim.x / kGroup;
+  // c is the same for every lane of a group, so groups stay whole.
+  for (uint32_t c = (blockIdx.x * blockDim.x + threadIdx.x) / kGroup; c < nc;
+       c += groups) {
+    uint32_t n = 0, first = 0xFFFFFFFFu, x0 = 0xFFFFFFFFu, x1 = 0,
+             y0 = 0xFFFFFFFFu, y1 = 0;
+    uint64_t peak = 0, sum = 0;
+    uint64_t sq[2] = {0, 0}, sx[2] = {0, 0}, sy[2] = {0, 0}, sxx[2] = {0, 0},
+             syy[2] = {0, 0};
     const uint32_t end = first_run[c + 1];
-    for (uint32_t j = first_run[c]; j < end; j++) {
+    for (uint32_t j = first_run[c] + sub; j < end; j += kGroup) {
       const run4 r = runs[order[j]]; // r0, x0, y, length
       const uint32_t start = r.z * f.width + r.y;
-      z.first = min(z.first, start);
-      z.x0 = min(z.x0, r.y);
-      z.x1 = max(z.x1, r.y + r.w);
-      z.y0 = min(z.y0, r.z);
-      z.y1 = max(z.y1, r.z + 1);
-      z.n += r.w;
-      uint64_t sum = 0;
+      first = min(first, start);
+      x0 = min(x0, r.y);
+      x1 = max(x1, r.y + r.w);
+      y0 = min(y0, r.z);
+      y1 = max(y1, r.z + 1);
+      n += r.w;
+      uint64_t run_sum = 0;
       for (uint32_t p = 0; p < r.w; p++) {
-        const uint64_t v = ordered[r.x + p].value;
+        const uint32_t v32 = ordered[r.x + p].value;
         comp_of[r.x + p] = c;
+        const uint64_t v = v32;
         const uint64_t ax = 2ull * (r.y + p) + 1ull;
         const uint64_t key = (v << 32) | (0xFFFFFFFFu - (start + p));
-        z.peak = key > z.peak ? key : z.peak;
-        sum += v;
-        acc128(z.sum_sq[0], z.sum_sq[1], v * v, 0);
-        acc128(z.sum_x[0], z.sum_x[1], v * ax, 0);
+        peak = key > peak ? key : peak;
+        run_sum += v;
+        acc128(sq[0], sq[1], v * v, 0);
+        acc128(sx[0], sx[1], v * ax, 0);
         const uint64_t xx = ax * ax;
-        acc128(z.sum_xx[0], z.sum_xx[1], v * xx, __umul64hi(v, xx));
+        acc128(sxx[0], sxx[1], v * xx, __umul64hi(v, xx));
       }
-      z.sum += sum;
+      sum += run_sum;
       // sum of value * ay and value * ay^2 over the run: ay is the run's
       // constant, so these are products of the run's sum.
       const uint64_t ay = 2ull * r.z + 1ull;
-      acc128(z.sum_y[0], z.sum_y[1], sum * ay, __umul64hi(sum, ay));
+      acc128(sy[0], sy[1], run_sum * ay, __umul64hi(run_sum, ay));
       const uint64_t yy = ay * ay;
-      acc128(z.sum_yy[0], z.sum_yy[1], sum * yy, __umul64hi(sum, yy));
+      acc128(syy[0], syy[1], run_sum * yy, __umul64hi(run_sum, yy));
     }
-    f.comps[c] = z;
+    auto add = [](auto a, auto b) { return a + b; };
+    auto lo = [](auto a, auto b) { return a < b ? a : b; };
+    auto hi = [](auto a, auto b) { return a > b ? a : b; };
+    n = group_reduce(n, add);
+    sum = group_reduce(sum, add);
+    peak = group_reduce(peak, hi);
+    first = group_reduce(first, lo);
+    x0 = group_reduce(x0, lo);
+    x1 = group_reduce(x1, hi);
+    y0 = group_reduce(y0, lo);
+    y1 = group_reduce(y1, hi);
+    group_add(sq[0], sq[1]);
+    group_add(sx[0], sx[1]);
+    group_add(sy[0], sy[1]);
+    group_add(sxx[0], sxx[1]);
+    group_add(syy[0], syy[1]);
+    if (sub == 0) {
+      sf_component z;
+      z.first = first;
+      z.n = n;
+      z.x0 = x0;
+      z.x1 = x1;
+      z.y0 = y0;
+      z.y1 = y1;
+      z.peak = peak;
+      z.sum = sum;
+      z.sum_sq[0] = sq[0], z.sum_sq[1] = sq[1];
+      z.sum_x[0] = sx[0], z.sum_x[1] = sx[1];
+      z.sum_y[0] = sy[0], z.sum_y[1] = sy[1];
+      z.sum_xx[0] = sxx[0], z.sum_xx[1] = sxx[1];
+      z.sum_yy[0] = syy[0], z.sum_yy[1] = syy[1];
+      f.comps[c] = z;
+    }
   }
 }
 

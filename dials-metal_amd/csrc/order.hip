@@ -61,26 +61,50 @@ __device__ __forceinline__ size_t live_words(const sfi::ScanSeg &s) {
   return w < s.words ? w : s.words;
 }
 
-// Per-block totals of popcounts; block (x, frame).  Thread t reads words t,
-// t + 256, ... of the block's share, so that each load instruction of a wave
-// covers 256 consecutive bytes.
+// Four words as one 16-byte access (a plain vector type: the HIP class
+// cannot be copied out of the global address space).
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
+
+// This thread's kPerThread consecutive words of the block's share of a bit
+// array, popcounted: four 16-byte loads where the share is whole, words past
+// the live ones reading as zero.
+__device__ __forceinline__ void thread_counts(const sfi::ScanSeg &s, size_t w0,
+                                              size_t words,
+                                              uint32_t (&c)[kPerThread]) {
+  const SF_GLOBAL uint32_t *bits = sfd::gmem(s.bits);
+  if (w0 + kPerThread <= words) {
+    const SF_GLOBAL u32x4 *v = reinterpret_cast<const SF_GLOBAL u32x4 *>(bits + w0);
+#pragma unroll
+    for (int i = 0; i < kPerThread / 4; i++) {
+      const u32x4 x = v[i];
+#pragma unroll
+      for (int k = 0; k < 4; k++)
+        c[4 * i + k] = static_cast<uint32_t>(__popc(x[k]));
+    }
+    return;
+  }
+#pragma unroll
+  for (int i = 0; i < kPerThread; i++)
+    c[i] = w0 + i < words ? static_cast<uint32_t>(__popc(bits[w0 + i])) : 0u;
+}
+
+// Per-block totals of popcounts; block (x, frame), each thread kPerThread
+// consecutive words.
 __global__ __launch_bounds__(kScanThreads) void popcount_totals(
     const sfi::ScanSeg *__restrict__ segs) {
   const sfi::ScanSeg s = segs[blockIdx.y];
-  const size_t words = live_words(s);
   const size_t base = static_cast<size_t>(blockIdx.x) * kScanPerBlock;
   if (base >= s.words)
     return;
-  const SF_GLOBAL uint32_t *bits = sfd::gmem(s.bits);
-  uint32_t c = 0;
+  uint32_t c[kPerThread];
+  thread_counts(s, base + static_cast<size_t>(threadIdx.x) * kPerThread,
+                live_words(s), c);
+  uint32_t sum = 0;
 #pragma unroll
-  for (int i = 0; i < kPerThread; i++) {
-    const size_t w = base + static_cast<size_t>(i) * kScanThreads + threadIdx.x;
-    if (w < words)
-      c += static_cast<uint32_t>(__popc(bits[w]));
-  }
+  for (int i = 0; i < kPerThread; i++)
+    sum += c[i];
   uint32_t total;
-  (void)block_inclusive(c, &total);
+  (void)block_inclusive(sum, &total);
   if (threadIdx.x == 0)
     s.totals[blockIdx.x] = total;
 }
@@ -103,52 +127,46 @@ __global__ __launch_bounds__(kScanThreads) void scan_totals(
   }
 }
 
-// Exclusive prefix of popcounts per word, frame-local.  Each wave takes 1024
-// consecutive words of the block's 4096, 64 at a time and coalesced (lane l
-// reads word 64 i + l of its share): a wave scan orders each step's counts
-// and a running count carries from step to step; the waves' starting counts
-// come from their totals, exchanged in LDS.
+// Exclusive prefix of popcounts per word, frame-local.  Each thread takes
+// kPerThread consecutive words of the block's share, a block scan of the
+// threads' sums gives each its start, and the thread writes its words'
+// prefixes (only those of words with a set bit, for a sparse array).
 __global__ __launch_bounds__(kScanThreads) void popcount_prefix(
     const sfi::ScanSeg *__restrict__ segs) {
-  constexpr uint32_t kWaveWords = kScanPerBlock / (kScanThreads / 64);
-  __shared__ uint32_t wave_totals[kScanThreads / 64];
   const sfi::ScanSeg s = segs[blockIdx.y];
   const size_t base = static_cast<size_t>(blockIdx.x) * kScanPerBlock;
   if (base >= s.words)
     return;
   const size_t words = live_words(s);
-  const SF_GLOBAL uint32_t *bits = sfd::gmem(s.bits);
+  const size_t w0 = base + static_cast<size_t>(threadIdx.x) * kPerThread;
+  uint32_t c[kPerThread];
+  thread_counts(s, w0, words, c);
+  uint32_t sum = 0;
+#pragma unroll
+  for (int i = 0; i < kPerThread; i++)
+    sum += c[i];
+  uint32_t total;
+  uint32_t run = s.totals[blockIdx.x] + block_inclusive(sum, &total) - sum;
   SF_GLOBAL uint32_t *prefix = sfd::gmem(s.prefix);
-  const uint32_t lane = threadIdx.x % 64, wave = threadIdx.x / 64;
-  const size_t share = base + static_cast<size_t>(wave) * kWaveWords;
-  uint32_t mine = 0;
-  for (uint32_t i = 0; i < kWaveWords; i += 64) {
-    const size_t w = share + i + lane;
-    if (w < words)
-      mine += static_cast<uint32_t>(__popc(bits[w]));
+  if (!s.sparse && w0 + kPerThread <= words) {
+    SF_GLOBAL u32x4 *v = reinterpret_cast<SF_GLOBAL u32x4 *>(prefix + w0);
+#pragma unroll
+    for (int i = 0; i < kPerThread / 4; i++) {
+      u32x4 x;
+#pragma unroll
+      for (int k = 0; k < 4; k++) {
+        x[k] = run;
+        run += c[4 * i + k];
+      }
+      v[i] = x;
+    }
+    return;
   }
 #pragma unroll
-  for (int d = 32; d >= 1; d >>= 1)
-    mine += __shfl_xor(mine, d);
-  if (lane == 0)
-    wave_totals[wave] = mine;
-  __syncthreads();
-  uint32_t run = s.totals[blockIdx.x];
-  for (uint32_t w = 0; w < wave; w++)
-    run += wave_totals[w];
-  for (uint32_t i = 0; i < kWaveWords; i += 64) {
-    const size_t w = share + i + lane;
-    const uint32_t c = w < words ? static_cast<uint32_t>(__popc(bits[w])) : 0u;
-    uint32_t incl = c;
-#pragma unroll
-    for (int d = 1; d < 64; d <<= 1) {
-      const uint32_t u = __shfl_up(incl, d);
-      if (lane >= static_cast<uint32_t>(d))
-        incl += u;
-    }
-    if (w < words)
-      prefix[w] = run + incl - c;
-    run += __shfl(incl, 63);
+  for (int i = 0; i < kPerThread; i++) {
+    if (w0 + i < words && (c[i] != 0 || !s.sparse))
+      prefix[w0 + i] = run;
+    run += c[i];
   }
 }
 

@@ -156,6 +156,9 @@ struct ScanSeg {
   size_t words;
   const uint32_t *live;
   uint32_t capacity;
+  // Nonzero: only words with a set bit get their prefix (every reader of a
+  // strong-pixel mask's prefix reads it for such words alone).
+  uint32_t sparse;
 };
 size_t scan_tmp_words(size_t words);
 // One scan per frame, all frames in three launches; descriptors on device.
