@@ -11,13 +11,13 @@
 //   * one wave per bitmask row counts the row's runs (a run starts at a set
 //     bit whose left neighbour is clear), a scan over the rows places each
 //     row's runs in the frame's run list, and a second pass lists each run
-//     {first rank, x0, y, length} and points its pixels at its first rank
-//     (no atomic: a returning atomic per row on one counter serialises the
-//     rows behind it);
-//   * each run is unioned with the runs above it that it touches, found as
-//     segments of the bitmask row above over its columns; a lock-free
-//     union-find hooks the larger root under the smaller, so the root of a
-//     component is its lowest rank, i.e. its first pixel;
+//     {first rank, x0, y, length} (no atomic: a returning atomic per row on
+//     one counter serialises the rows behind it);
+//   * each run, named by its first rank, is unioned with the runs above it
+//     that it touches, found as segments of the bitmask row above over its
+//     columns and traced left in that row to their runs' first pixels; a
+//     lock-free union-find hooks the larger root under the smaller, so the
+//     root of a component is its lowest rank, i.e. its first pixel;
 //   * roots are numbered by a popcount scan of a root bitmask, which numbers
 //     components in order of first appearance -- the reference's order;
 //   * the runs are sorted by component (one atomic per run takes its place,
@@ -108,17 +108,12 @@ __device__ __forceinline__ uint32_t columns(uint32_t j, uint32_t x0, uint32_t x1
   return upto & ~((1u << lo) - 1u);
 }
 
-// Wave-wide exclusive prefix of v and the wave's total.
+// Wave-wide exclusive prefix of v and the wave's total (every lane active).
 __device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t lane,
                                                 uint32_t *total) {
-  uint32_t incl = v;
-#pragma unroll
-  for (int d = 1; d < kLanes; d <<= 1) {
-    const uint32_t o = __shfl_up(incl, d);
-    if (lane >= static_cast<uint32_t>(d))
-      incl += o;
-  }
-  *total = __shfl(incl, kLanes - 1);
+  (void)lane;
+  const uint32_t incl = sfd::wave_inclusive(v);
+  *total = sfd::last_lane(incl);
   return incl - v;
 }
 
@@ -230,15 +225,19 @@ __global__ __launch_bounds__(1024) void scan_runs(const FrameDev *__restrict__ f
   }
   if (threadIdx.x == 0)
     f.counter[4] = all;
+  // The root bitmask's live words start clear: flatten sets the roots' bits.
+  const uint32_t root_words = (live_count(f) + 31u) / 32u;
+  for (uint32_t w = threadIdx.x; w < root_words; w += blockDim.x)
+    gmem(f.roots)[w] = 0;
 }
 
 // The runs of every frame, one wave per bitmask row, each row's at its place
-// in the frame's run list; every pixel of a run points at the run's first
-// rank (the union-find's initial forest).  A row's words and rank prefixes
-// are loaded in one round (lane l holds words l, l + 64, ...), neighbouring
+// in the frame's run list; each run's first rank is a root of the
+// union-find's initial forest (the union-find works on runs alone, each
+// named by its first rank).  A row's base, words and rank prefixes are
+// loaded in one round (lane l holds words l, l + 64, ...), neighbouring
 // words come from the neighbouring lanes, and only a run that crosses a whole
-// word reads the row again.  Rows without runs are skipped from the row
-// scan alone.
+// word reads the row again.
 __global__ __launch_bounds__(256) void emit_runs(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   if (live_count(f) == 0)
@@ -246,16 +245,14 @@ __global__ __launch_bounds__(256) void emit_runs(const FrameDev *__restrict__ fr
   const uint32_t lane = threadIdx.x % kLanes;
   const uint32_t per_block = blockDim.x / kLanes;
   const uint32_t wpr = f.wpr;
-  const uint32_t nr = f.counter[4];
   const SF_GLOBAL uint32_t *bases = gmem(f.row_runs);
   SF_GLOBAL uint32_t *parent = gmem(f.parent);
   SF_GLOBAL run4 *runs = run_list(f);
   for (uint32_t y = blockIdx.x * per_block + threadIdx.x / kLanes; y < f.height;
        y += gridDim.x * per_block) {
+    // (The row's base is read with its words, in the same round: a row
+    // without runs only costs that round.)
     uint32_t k = bases[y];
-    const uint32_t next = y + 1 < f.height ? bases[y + 1] : nr;
-    if (next == k)
-      continue;
     const SF_GLOBAL uint32_t *row = gmem(f.mask) + static_cast<size_t>(y) * wpr;
     const SF_GLOBAL uint32_t *pre = gmem(f.prefix) + static_cast<size_t>(y) * wpr;
     uint32_t carry_in = 0; // bit 31 of the word before this round's first
@@ -304,9 +301,10 @@ __global__ __launch_bounds__(256) void emit_runs(const FrameDev *__restrict__ fr
               nw = row[jj];
             }
           }
-          runs[at++] = run4{r0, 32u * j + b, y, len};
-          for (uint32_t r = r0; r < r0 + len; r++)
-            parent[r] = r0;
+          if (at < f.capacity) // always: runs never outnumber pixels
+            runs[at] = run4{r0, 32u * j + b, y, len};
+          at++;
+          parent[r0] = r0;
         }
         k += total;
       }
@@ -315,8 +313,33 @@ __global__ __launch_bounds__(256) void emit_runs(const FrameDev *__restrict__ fr
   }
 }
 
+// Rank of the first pixel of the run that holds bit b of word j of bitmask
+// row `row` (a set bit; `full` is that word), `pre` the row's rank prefixes:
+// after the last clear bit before it, found in this word or, for a run that
+// reaches the word's bit 0, in the words to its left.
+__device__ uint32_t run_start_rank(const SF_GLOBAL uint32_t *row,
+                                   const SF_GLOBAL uint32_t *pre, uint32_t j,
+                                   uint32_t b, uint32_t full) {
+  const uint32_t clear = ~full & ((1u << b) - 1u);
+  if (clear != 0) {
+    const uint32_t s = 32u - static_cast<uint32_t>(__clz(clear));
+    return pre[j] + static_cast<uint32_t>(__popc(full & ((1u << s) - 1u)));
+  }
+  for (; j > 0; j--) {
+    const uint32_t w = row[j - 1];
+    if (w == 0xFFFFFFFFu)
+      continue;
+    if ((w >> 31) == 0) // the run starts at bit 0 of word j
+      return pre[j];
+    const uint32_t s = 32u - static_cast<uint32_t>(__clz(~w));
+    return pre[j - 1] + static_cast<uint32_t>(__popc(w & ((1u << s) - 1u)));
+  }
+  return pre[0]; // the run starts the row
+}
+
 // Unites each run with the runs above it that share a column with it: one
-// union per segment of the row above within the run's columns.
+// union per segment of the row above within the run's columns, with the run
+// that holds the segment.
 __global__ __launch_bounds__(256) void unite_runs(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   const uint32_t nr = f.counter[4];
@@ -340,41 +363,35 @@ __global__ __launch_bounds__(256) void unite_runs(const FrameDev *__restrict__ f
       while (segs) {
         const uint32_t b = static_cast<uint32_t>(__ffs(segs)) - 1u;
         segs &= segs - 1u;
-        unite(parent, r.x,
-              pre[j] + static_cast<uint32_t>(__popc(full & ((1u << b) - 1u))));
+        unite(parent, r.x, run_start_rank(row, pre, j, b, full));
       }
     }
   }
 }
 
-// Flatten every tree to its root; mark the roots in a bitmask of ranks.  A
-// wave covers 64 consecutive ranks from a multiple of 64, so its ballot is
-// two whole words, and the words up to the live count are all written.
+// Flatten every run's tree to its root and mark the roots (runs that are
+// their own root, each the first pixel of its component) in the bitmask of
+// ranks, one atomic per root.
 __global__ __launch_bounds__(256) void flatten(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
-  const uint32_t n = live_count(f);
-  const uint32_t span = (n + 63u) & ~63u;
+  const uint32_t nr = f.counter[4];
   SF_GLOBAL uint32_t *parent = gmem(f.parent);
-  for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < span;
-       i += gridDim.x * blockDim.x) {
-    bool root = false;
-    if (i < n) {
-      uint32_t r = load_relaxed(&parent[i]);
-      for (;;) {
-        const uint32_t up = load_relaxed(&parent[r]);
-        if (up == r)
-          break;
-        r = up;
-      }
-      store_relaxed(&parent[i], r);
-      root = r == i;
+  const SF_GLOBAL run4 *runs = run_list(f);
+  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
+       k += gridDim.x * blockDim.x) {
+    const uint32_t i = runs[k].x;
+    uint32_t r = load_relaxed(&parent[i]);
+    for (;;) {
+      const uint32_t up = load_relaxed(&parent[r]);
+      if (up == r)
+        break;
+      r = up;
     }
-    const uint64_t b = __ballot(root);
-    const uint32_t lane = sfd::lane_id();
-    if (lane == 0)
-      gmem(f.roots)[i / 32] = static_cast<uint32_t>(b);
-    else if (lane == 32)
-      gmem(f.roots)[i / 32] = static_cast<uint32_t>(b >> 32);
+    if (r == i)
+      __hip_atomic_fetch_or(gmem(f.roots) + i / 32, 1u << (i & 31u),
+                            __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    else
+      store_relaxed(&parent[i], r);
   }
 }
 
@@ -399,7 +416,8 @@ __global__ __launch_bounds__(256) void open_components(
     if (parent[r.x] == r.x)
       gmem(f.comp_runs)[component_of(f, r.x)] = 0;
     if (r.x + r.w == n)
-      f.counter[2] = component_of(f, n - 1) + (parent[n - 1] == n - 1 ? 1u : 0u);
+      f.counter[2] = component_of(f, n - 1) +
+                     ((gmem(f.roots)[(n - 1) / 32] >> ((n - 1) & 31u)) & 1u);
   }
 }
 
@@ -468,8 +486,11 @@ __global__ __launch_bounds__(256) void place_runs(const FrameDev *__restrict__ f
   const SF_GLOBAL uint32_t *first = gmem(f.comp_runs);
   SF_GLOBAL uint32_t *order = gmem(f.parent);
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
-       k += gridDim.x * blockDim.x)
-    order[first[scratch[2 * k]] + scratch[2 * k + 1]] = k;
+       k += gridDim.x * blockDim.x) {
+    const uint32_t at = first[scratch[2 * k]] + scratch[2 * k + 1];
+    if (at < nr) // always: the places are a permutation of the runs
+      order[at] = k;
+  }
 }
 
 // Adds a 128-bit value held as (lo, hi) to a running (lo, hi) in registers.

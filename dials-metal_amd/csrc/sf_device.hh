@@ -82,6 +82,33 @@ __device__ __forceinline__ uint32_t lane_id() {
   return __builtin_amdgcn_mbcnt_hi(~0u, __builtin_amdgcn_mbcnt_lo(~0u, 0u));
 }
 
+// v from a DPP source lane, 0 where the source is outside the row or the
+// row is not in `RowMask` (the destination keeps the old value, 0).
+template <int Ctrl, int RowMask>
+__device__ __forceinline__ uint32_t dpp_or_zero(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(
+      0, static_cast<int>(v), Ctrl, RowMask, 0xf, false));
+}
+
+// Inclusive sum of v over lanes 0 .. this lane of the wave (every lane
+// active), all in VALU lane moves: shifts by 1, 2, 4 and 8 within each row of
+// 16 lanes (row_shr), then lane 15 of each row added to the next row and lane
+// 31 to the upper half (row_bcast:15, row_bcast:31).
+__device__ __forceinline__ uint32_t wave_inclusive(uint32_t v) {
+  v += dpp_or_zero<0x111, 0xf>(v); // row_shr:1
+  v += dpp_or_zero<0x112, 0xf>(v); // row_shr:2
+  v += dpp_or_zero<0x114, 0xf>(v); // row_shr:4
+  v += dpp_or_zero<0x118, 0xf>(v); // row_shr:8
+  v += dpp_or_zero<0x142, 0xa>(v); // row_bcast:15 into rows 1 and 3
+  v += dpp_or_zero<0x143, 0xc>(v); // row_bcast:31 into rows 2 and 3
+  return v;
+}
+
+// The value of lane 63.
+__device__ __forceinline__ uint32_t last_lane(uint32_t v) {
+  return static_cast<uint32_t>(__builtin_amdgcn_readlane(static_cast<int>(v), 63));
+}
+
 // Appends this lane's record if `signal`, one atomic per wave.  Every lane of
 // the wave must call it.  Writes past `capacity` are dropped but still
 // counted, so the caller can report the overflow (dext_cuda.cu:294-297).
