@@ -540,34 +540,80 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       scr0 |= ((mh[h] >> 12) & 0xFu) << (4 * h);
       m7any |= mh[h] & 0x0FFFFFFFu;
     }
-    {
+    // A masked pixel in rows y_in-6 .. y_in of any lane: counts vary and
+    // travel in the top bits of the sums; otherwise every count is 49.
+    if (__ballot(m7any != 0) == 0) {
+      // Lattice first: the columns 4k + 2 of every lane.  Every 5 x 5 erode
+      // window that reaches an output through stage 2 lies within the strip
+      // and holds one such column in each of its rows, and here no pixel of
+      // the row is masked; so when every lattice pixel of the row tests
+      // negative, no such window is all scr0 whatever the row's other
+      // pixels test, and they are left clear.  Otherwise the other columns'
+      // sums follow from the lattice columns' by the columns entering and
+      // leaving.  Column i of the row (i from -3 to KQ + 2, the outer ones
+      // the neighbouring lanes') is at [i + 3].
+      double es[KQ + 6];
+      uint32_t ew[KQ + 6];
+#pragma unroll
+      for (int q = 0; q < KQ; q++) {
+        es[q + 3] = v0s[q];
+        ew[q + 3] = v0w[q];
+      }
+      es[2] = from_left(v0s[KQ - 1]);
+      ew[2] = from_left(v0w[KQ - 1]);
+      es[KQ + 3] = from_right(v0s[0]);
+      ew[KQ + 3] = from_right(v0w[0]);
+      es[KQ + 4] = from_right(v0s[1]);
+      ew[KQ + 4] = from_right(v0w[1]);
+      double hs[KQ];
+      uint32_t hw[KQ];
+      uint32_t lattice = 0;
+#pragma unroll
+      for (int q = 2; q < KQ; q += 4) {
+        hs[q] = tree_sum<7>(es + q);
+        hw[q] = tree_sum<7>(ew + q);
+        if (stage0_test<T>(49u, hw[q], hs[q], root49))
+          lattice |= 1u << q;
+      }
+      if (__ballot(lattice != 0) != 0) {
+        es[1] = from_left(v0s[KQ - 2]);
+        ew[1] = from_left(v0w[KQ - 2]);
+        es[0] = from_left(v0s[KQ - 3]);
+        ew[0] = from_left(v0w[KQ - 3]);
+        es[KQ + 5] = from_right(v0s[2]);
+        ew[KQ + 5] = from_right(v0w[2]);
+#pragma unroll
+        for (int q = 2; q < KQ; q += 4) {
+          hs[q - 1] = (hs[q] - es[q + 6]) + es[q - 1];
+          hw[q - 1] = (hw[q] - ew[q + 6]) + ew[q - 1];
+          hs[q - 2] = (hs[q - 1] - es[q + 5]) + es[q - 2];
+          hw[q - 2] = (hw[q - 1] - ew[q + 5]) + ew[q - 2];
+          hs[q + 1] = (hs[q] - es[q]) + es[q + 7];
+          hw[q + 1] = (hw[q] - ew[q]) + ew[q + 7];
+        }
+        scr0 |= lattice;
+#pragma unroll
+        for (int q = 0; q < KQ; q++)
+          if ((q & 3) != 2 && stage0_test<T>(49u, hw[q], hs[q], root49))
+            scr0 |= 1u << q;
+      }
+    } else {
       double hs[KQ];
       across<3>(v0s, hs);
-      // A masked pixel in rows y_in-6 .. y_in of any lane: counts vary and
-      // travel in the top bits of the sums; otherwise every count is 49.
-      if (__ballot(m7any != 0) == 0) {
-        uint32_t hw[KQ];
-        across<3>(v0w, hw);
+      uint32_t word[KQ], hw[KQ];
 #pragma unroll
-        for (int q = 0; q < KQ; q++)
-          if (stage0_test<T>(49u, hw[q], hs[q], root49))
-            scr0 |= 1u << q;
-      } else {
-        uint32_t word[KQ], hw[KQ];
+      for (int q = 0; q < KQ; q++)
+        word[q] = v0w[q] +
+                  ((7u - static_cast<uint32_t>(__popc(
+                             mh[q / 4] &
+                             static_cast<uint32_t>(column_bits(0, 7, q % 4)))))
+                   << 26);
+      across<3>(word, hw);
 #pragma unroll
-        for (int q = 0; q < KQ; q++)
-          word[q] = v0w[q] +
-                    ((7u - static_cast<uint32_t>(__popc(
-                               mh[q / 4] &
-                               static_cast<uint32_t>(column_bits(0, 7, q % 4)))))
-                     << 26);
-        across<3>(word, hw);
-#pragma unroll
-        for (int q = 0; q < KQ; q++) {
-          const uint32_t m = hw[q] >> 26;
-          if (stage0_test<T>(m, hw[q] & (kCount0 - 1), hs[q], table[m]))
-            scr0 |= 1u << q;
-        }
+      for (int q = 0; q < KQ; q++) {
+        const uint32_t m = hw[q] >> 26;
+        if (stage0_test<T>(m, hw[q] & (kCount0 - 1), hs[q], table[m]))
+          scr0 |= 1u << q;
       }
     }
 
