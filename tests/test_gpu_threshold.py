@@ -155,6 +155,56 @@ def test_fused_fast_path_wide_values(device, oracle_lib):
     ws.close()
 
 
+def striped_frame(rng, h, w, dt, phase, width, mask_phase):
+    """Poisson background; bright vertical stripes `width` columns wide and
+    up to 40 rows long every 23 columns from column `phase`, and a few bright
+    bars across the frame; masked columns every 37 from `mask_phase` and
+    masked rows every 41.  Stage-0 positives then fall on and off any lattice
+    of columns with every phase, beside masked columns of every phase."""
+    top = np.iinfo(dt).max
+    f = rng.poisson(2.0, (h, w)).astype(np.int64)
+    for x0 in range(phase, w, 23):
+        cols = slice(x0, min(w, x0 + width))
+        for y in rng.integers(0, h, 3):
+            rows = slice(y, min(h, y + rng.integers(3, 40)))
+            f[rows, cols] += rng.integers(20, 200)
+    for y in rng.integers(0, h, max(1, h // 100)):
+        f[y, :] += rng.integers(20, 100)
+    f = np.minimum(f, top - 2)
+    f[:, mask_phase::37] = top - 1
+    f[5::41, :] = top
+    return f.astype(dt)
+
+
+@pytest.mark.parametrize("lane_columns", [4, 8])
+def test_lane_widths_and_lattice_against_oracle(device, oracle_lib, lane_columns):
+    """The fused kernel with four and with eight columns a lane, bit-exact
+    against dext on frames built to exercise its lattice pre-test (stage-0
+    positives that miss the lattice columns of some rows) and its counted
+    path (masked columns at every lattice phase), on shapes that split into
+    several strips, and on a 4M EIGER frame."""
+    start = device.get_tuning()
+    device.set_tuning(lane_columns=lane_columns)
+    try:
+        rng = np.random.default_rng(1000 + lane_columns)
+        frames = []
+        for (h, w) in [(300, 1000), (257, 517), (64, 2100)]:
+            for dt in (np.uint16, np.uint32):
+                for phase, width in [(0, 1), (1, 2), (2, 3), (3, 4), (5, 2)]:
+                    frames.append(((h, w, dt.__name__, phase, width),
+                                   striped_frame(rng, h, w, dt, phase, width,
+                                                 (phase * 7) % 37)))
+        frames.append(("4m synth", device.synth_frame(device.synth_desc("4m"), 1)))
+        for name, f in frames:
+            say("lane columns", lane_columns, name)
+            rc, got = device.find(f)
+            _, want = oracle_lib.dext(f)
+            assert len(want) <= f.nbytes // 16, name  # within the list's capacity
+            assert rc == 0 and got.tobytes() == want.tobytes(), name
+    finally:
+        device.set_tuning(**start)
+
+
 def test_stage_times(device):
     device.profile_stages(True)
     try:
