@@ -10,8 +10,9 @@ with exact per-component moments (2D), all on the device.
 
 Frames are generated on the host, uploaded once and resident in HBM before
 the timed region.  A step is one batch of frames (--frames) through one
-persistent threshold launch, the batched ordering and the batched labelling;
-`value` is frames per second over the timed steps.  Beside it: the plain
+persistent threshold launch, the batched ordering and the batched labelling,
+waited for before the next (--in-flight 2 alternates two batch handles on two
+streams instead); `value` is frames per second over the timed steps.  Beside it: the plain
 per-frame mode (one frame per launch sequence, waiting for each), the fused
 kernel's achieved HBM bandwidth against the 8 TB/s peak from HIP events on
 its stream, and the reference's own CPU threshold (compiled from its
@@ -194,6 +195,9 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=32,
                     help="frames per batch (one step), resident in HBM")
+    ap.add_argument("--in-flight", type=int, default=1,
+                    help="batch handles queued at once in the timed steps "
+                         "(each on its own stream)")
     ap.add_argument("--graph", type=int, default=1,
                     help="replay the batch's threshold from a captured hipGraph")
     ap.add_argument("--no-label", action="store_true")
@@ -236,24 +240,38 @@ def main():
         tuned = sf.autotune(descs, label=not args.no_label)
         log(f"autotune: {tuned}")
 
-    batch = sf.Batch()
-    counts = []
+    # Batch handles in flight (one by default): each has its own stream and
+    # slots, so with two the next step is queued before the last one is
+    # waited for; every step's results are still waited for and checked.
+    batches = [sf.Batch() for _ in range(max(1, args.in_flight))]
+    batch = batches[0]
 
-    def step(graph=True):
-        batch.threshold(descs, graph=graph)
+    def submit(b):
+        b.threshold(descs, graph=bool(args.graph))
         if label:
-            batch.label(True)
-        c, status = batch.sync()
+            b.label(True)
+
+    def finish(b):
+        c, status = b.sync()
         if any(status):
             raise RuntimeError(f"a frame was refused: {status}")
         return c
 
-    for _ in range(args.warmup):
-        counts = step(bool(args.graph))
+    def run(n):
+        c = []
+        k = len(batches)
+        for i in range(n):
+            if i >= k:
+                c = finish(batches[i % k])
+            submit(batches[i % k])
+        for i in range(max(0, n - k), n):
+            c = finish(batches[i % k])
+        return c
+
+    counts = run(args.warmup)
     sf.synchronize()
     t0 = time.perf_counter()
-    for _ in range(args.steps):
-        step(bool(args.graph))
+    run(args.steps)
     sf.synchronize()
     elapsed = time.perf_counter() - t0
     frames_done = args.steps * len(dev)
@@ -269,7 +287,9 @@ def main():
     kernel_ms = float(np.median(kms))
     # Frames re-run on the exact direct kernels because a valid pixel was too
     # large for the fused kernel's fast sums; on this workload, none.
-    fallbacks = batch.fallbacks()
+    fallbacks = sum(b.fallbacks() for b in batches)
+    for b in batches:  # their slots are not needed by the sections below
+        b.close()
     mean_signal = float(np.mean(counts))
     n_px = h * w
     bytes_per_frame = n_px * 4 + 16 * mean_signal + n_px / 8.0
@@ -296,7 +316,8 @@ def main():
                         + (", 2D labelling with exact per-component moments"
                            if label else ""),
             "frame": [h, w], "pixel": "uint32", "frames_per_step": len(dev),
-            "graph": bool(args.graph), "tuning": sf.get_tuning(),
+            "graph": bool(args.graph), "batches_in_flight": len(batches),
+            "tuning": sf.get_tuning(),
             "autotune": tuned,
             "mean_signal_pixels": mean_signal},
         "roofline": {

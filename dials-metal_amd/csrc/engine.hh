@@ -133,7 +133,6 @@ private:
   // The captured batch, and what it was captured for.
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
-  std::vector<int> graph_ids_;
   std::vector<uint64_t> graph_key_;
 };
 

@@ -327,14 +327,18 @@ void Engine::threshold(const FrameIn *frames, const int *ids, size_t n,
   if (timing && !marks_[0])
     for (hipEvent_t &e : marks_)
       check(hipEventCreate(&e), "hipEventCreate");
-  if (!graph || timing) {
+  // Only the fused sequence is captured: the direct kernels take their frame
+  // and slot pointers as launch arguments, which a replay would repeat.
+  if (!graph || timing || kernel != SF_KERNEL_FUSED) {
     const Prepared p = prepare(frames, ids, n, kernel);
     enqueue_kernels(p, frames, ids, n, kernel, timing);
     return;
   }
-  // One graph per (shapes, slots, tuning, buffer epoch): a replay runs the
-  // same kernels over the same buffers, reading this batch's frame pointers
-  // from the descriptors uploaded just before it, outside the graph.
+  // One graph per (shapes, tuning, buffer epoch): every frame and slot
+  // pointer the fused sequence uses is read from the descriptors uploaded
+  // just before each launch, outside the graph, and the engine buffers it
+  // names change the epoch when they move; so one replay serves any batch of
+  // these shapes, in either set of slots.
   const sf_tuning t = tuning();
   std::vector<uint64_t> key = {static_cast<uint64_t>(kernel), epoch_,
                                static_cast<uint64_t>(t.seg_rows),
@@ -345,9 +349,8 @@ void Engine::threshold(const FrameIn *frames, const int *ids, size_t n,
     key.push_back((static_cast<uint64_t>(frames[i].height) << 32) |
                   (static_cast<uint64_t>(frames[i].width) << 8) |
                   static_cast<uint64_t>(frames[i].bits));
-  const std::vector<int> id_list(ids, ids + n);
   const Prepared p = prepare(frames, ids, n, kernel);
-  if (!(graph_exec_ && key == graph_key_ && id_list == graph_ids_)) {
+  if (!(graph_exec_ && key == graph_key_)) {
     if (graph_exec_)
       check(hipGraphExecDestroy(graph_exec_), "hipGraphExecDestroy");
     if (graph_)
@@ -361,7 +364,6 @@ void Engine::threshold(const FrameIn *frames, const int *ids, size_t n,
     check(hipGraphInstantiate(&graph_exec_, graph_, nullptr, nullptr, 0),
           "hipGraphInstantiate");
     graph_key_ = key;
-    graph_ids_ = id_list;
   }
   check(hipGraphLaunch(graph_exec_, stream_), "hipGraphLaunch");
 }

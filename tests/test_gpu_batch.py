@@ -69,11 +69,23 @@ def run_batch(device, batch, frames, graph):
     return dev
 
 
-@pytest.mark.parametrize("graph", [False, True])
-def test_mixed_batch_bit_exact(device, synth_cases, matrix, graph):
+@pytest.mark.parametrize("graph,kernel", [(False, "fused"), (True, "fused"),
+                                          (True, "direct")])
+def test_mixed_batch_bit_exact(device, synth_cases, matrix, graph, kernel):
+    """With graphs on, each replay takes new frame arrays of the same shapes:
+    the fused sequence reads every pointer from the descriptors, and the
+    direct kernels, which take theirs as launch arguments, are not captured."""
     frames = mixed_frames(device, synth_cases, matrix)
-    batch = device.Batch()
-    say("mixed batch", len(frames), "graph", graph)
+    device.set_kernel(kernel)
+    try:
+        batch = device.Batch()
+        say("mixed batch", len(frames), "graph", graph, kernel)
+        replay_mixed(device, batch, frames, graph)
+    finally:
+        device.set_kernel("fused")
+
+
+def replay_mixed(device, batch, frames, graph):
     run_batch(device, batch, frames, graph)
     if graph:
         # Replays with other frames of the same shapes, eager work between.
