@@ -504,6 +504,8 @@ __device__ __forceinline__ void acc128(uint64_t &lo, uint64_t &hi, uint64_t l,
 // Lanes that sum one component together: a component has up to a few tens
 // of runs of a few pixels each.
 constexpr uint32_t kGroup = 8;
+// Pixel values a lane loads at once.
+constexpr uint32_t kPixelBatch = 4;
 
 // Totals over the kGroup lanes of a group (lanes g .. g + kGroup - 1, g a
 // multiple of kGroup), every lane of the group getting them.
@@ -545,8 +547,14 @@ __global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ f
     uint64_t sq[2] = {0, 0}, sx[2] = {0, 0}, sy[2] = {0, 0}, sxx[2] = {0, 0},
              syy[2] = {0, 0};
     const uint32_t end = first_run[c + 1];
-    for (uint32_t j = first_run[c] + sub; j < end; j += kGroup) {
-      const run4 r = runs[order[j]]; // r0, x0, y, length
+    uint32_t j = first_run[c] + sub;
+    run4 next = {0u, 0u, 0u, 0u};
+    if (j < end)
+      next = runs[order[j]];
+    for (; j < end; j += kGroup) {
+      const run4 r = next; // r0, x0, y, length
+      if (j + kGroup < end) // the lane's next run, loaded during this one
+        next = runs[order[j + kGroup]];
       const uint32_t start = r.z * f.width + r.y;
       first = min(first, start);
       x0 = min(x0, r.y);
@@ -555,18 +563,28 @@ __global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ f
       y1 = max(y1, r.z + 1);
       n += r.w;
       uint64_t run_sum = 0;
-      for (uint32_t p = 0; p < r.w; p++) {
-        const uint32_t v32 = ordered[r.x + p].value;
-        comp_of[r.x + p] = c;
-        const uint64_t v = v32;
-        const uint64_t ax = 2ull * (r.y + p) + 1ull;
-        const uint64_t key = (v << 32) | (0xFFFFFFFFu - (start + p));
-        peak = key > peak ? key : peak;
-        run_sum += v;
-        acc128(sq[0], sq[1], v * v, 0);
-        acc128(sx[0], sx[1], v * ax, 0);
-        const uint64_t xx = ax * ax;
-        acc128(sxx[0], sxx[1], v * xx, __umul64hi(v, xx));
+      // The run's values, kPixelBatch loads in flight at a time.
+      for (uint32_t p0 = 0; p0 < r.w; p0 += kPixelBatch) {
+        uint32_t vals[kPixelBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kPixelBatch; k++)
+          vals[k] = p0 + k < r.w ? ordered[r.x + p0 + k].value : 0u;
+#pragma unroll
+        for (uint32_t k = 0; k < kPixelBatch; k++) {
+          const uint32_t p = p0 + k;
+          if (p >= r.w)
+            break;
+          comp_of[r.x + p] = c;
+          const uint64_t v = vals[k];
+          const uint64_t ax = 2ull * (r.y + p) + 1ull;
+          const uint64_t key = (v << 32) | (0xFFFFFFFFu - (start + p));
+          peak = key > peak ? key : peak;
+          run_sum += v;
+          acc128(sq[0], sq[1], v * v, 0);
+          acc128(sx[0], sx[1], v * ax, 0);
+          const uint64_t xx = ax * ax;
+          acc128(sxx[0], sxx[1], v * xx, __umul64hi(v, xx));
+        }
       }
       sum += run_sum;
       // sum of value * ay and value * ay^2 over the run: ay is the run's
