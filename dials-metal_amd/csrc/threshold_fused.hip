@@ -643,21 +643,26 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
 
     // ---- stage 2: vertical 11-row sums of values outside scr1 ----
     {
-      if ((x1_rows & ((1u << 0) | (1u << 11))) == 0) {
-        // no unmasked scr1 pixel in the wave in rows y_in-5 and y_in-16
+      // The entering row's value less the leaving row's, then the two
+      // corrections, each only where the wave-uniform record says its row
+      // holds an unmasked scr1 pixel: an entering one must not be added, a
+      // leaving one was never added (masked pixels are zero anyway).
+#pragma unroll
+      for (int q = 0; q < KQ; q++)
+        v2w[q] += d2[q];
+      if (x1_rows & 1u) {
 #pragma unroll
         for (int q = 0; q < KQ; q++)
-          v2w[q] += d2[q];
-      } else {
-        // Row y_in-16's exclusion bits; its masked pixels are zero anyway.
-        uint32_t xout = 0;
+          v2w[q] -= ((x1 >> q) & 1u) ? in2[q] : 0u;
+      }
+      if ((x1_rows >> 11) & 1u) {
+        uint32_t xout = 0; // row y_in-16's exclusion bits
 #pragma unroll
         for (int h = 0; h < P; h++)
           xout |= (static_cast<uint32_t>(xh[h] >> 40) & 0xFu) << (4 * h);
 #pragma unroll
         for (int q = 0; q < KQ; q++)
-          v2w[q] += (((x1 >> q) & 1u) ? 0u : in2[q]) -
-                    (((xout >> q) & 1u) ? 0u : out2[q]);
+          v2w[q] += ((xout >> q) & 1u) ? out2[q] : 0u;
       }
 #pragma unroll
       for (int h = 0; h < P; h++)
@@ -690,15 +695,36 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       ring_row(s10, raw);
       if ((wide_rows >> 10) & 1u)
         add_high<KQ>(high[s10][lane], raw);
+      uint32_t cand = 0;
 #pragma unroll
-      for (int q = 0; q < KQ; q++) {
+      for (int q = 0; q < KQ; q++)
+        cand |= (((maybe >> q) & 1u) && raw[q] > 0 ? 1u : 0u) << q;
+      auto entry = [&](int q) {
         const int c = c0 + q;
-        const bool cand = ((maybe >> q) & 1u) && raw[q] > 0;
-        const uint4 entry = make_uint4(
+        return make_uint4(
             static_cast<uint32_t>(y2) * a.width + static_cast<uint32_t>(c),
             raw[q] | (static_cast<uint32_t>(c & 31) << 18), hsum[q],
             static_cast<uint32_t>(y2) * a.wpr + static_cast<uint32_t>(c) / 32);
-        put(cand, entry);
+      };
+      // One wave scan of the lanes' candidate counts places them all when
+      // the stage has room; a row with more goes column by column through
+      // put's chunks.
+      const uint32_t k = static_cast<uint32_t>(__popc(cand));
+      const uint32_t incl = wave_inclusive(k);
+      const uint32_t total = last_lane(incl);
+      if (staged + total <= kStage) {
+        uint32_t at = staged + incl - k;
+#pragma unroll
+        for (int q = 0; q < KQ; q++)
+          if ((cand >> q) & 1u)
+            stage[at++] = entry(q);
+        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+        __builtin_amdgcn_wave_barrier();
+        staged += total;
+      } else {
+#pragma unroll
+        for (int q = 0; q < KQ; q++)
+          put(((cand >> q) & 1u) != 0, entry(q));
       }
     }
     slot = (slot + 1) & (kRing - 1);
