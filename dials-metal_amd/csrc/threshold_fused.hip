@@ -601,12 +601,13 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       double hs[KQ];
       across<3>(v0s, hs);
       uint32_t word[KQ], hw[KQ];
+      // each column's unmasked pixels in rows y_in-6 .. y_in, counted
+      // directly from the clear bits of the masked history
 #pragma unroll
       for (int q = 0; q < KQ; q++)
         word[q] = v0w[q] +
-                  ((7u - static_cast<uint32_t>(__popc(
-                             mh[q / 4] &
-                             static_cast<uint32_t>(column_bits(0, 7, q % 4)))))
+                  (static_cast<uint32_t>(__popc(
+                       ~mh[q / 4] & static_cast<uint32_t>(column_bits(0, 7, q % 4))))
                    << 26);
       across<3>(word, hw);
 #pragma unroll
@@ -680,14 +681,14 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       for (int h = 0; h < P; h++)
         maybe |= (static_cast<uint32_t>(xh[h] >> 20) & 0xFu) << (4 * h);
       maybe &= in_out;
-      // Counts: rows y_in-15 .. y_in-5, less the excluded.
+      // Counts: the pixels of rows y_in-15 .. y_in-5 not excluded, from the
+      // clear bits of the exclusion history.
       uint32_t word[KQ];
 #pragma unroll
-      for (int q = 0; q < KQ; q++) {
-        const uint32_t excluded = static_cast<uint32_t>(
-            __popcll(xh[q / 4] & column_bits(0, 11, q % 4)));
-        word[q] = v2w[q] + ((11u - excluded) << 25);
-      }
+      for (int q = 0; q < KQ; q++)
+        word[q] = v2w[q] + (static_cast<uint32_t>(__popcll(
+                                ~xh[q / 4] & column_bits(0, 11, q % 4)))
+                            << 25);
       uint32_t hsum[KQ];
       across<5>(word, hsum);
       const int s10 = (slot - 10) & (kRing - 1);
