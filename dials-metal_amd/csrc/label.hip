@@ -630,10 +630,14 @@ __global__ __launch_bounds__(256) void accumulate(const FrameDev *__restrict__ f
   }
 }
 
+// Links a lane keeps in registers while its run is walked.
+constexpr uint32_t kLinksHeld = 4;
+
 // Pairs (component here, component on the previous frame) for the pixels set
 // in both frames: per run, the previous frame's components along its columns,
-// each pair once per run (a component repeated along the run is listed once).
-// One atomic per wave on the frame's link count.
+// in column order, a component met again straight after itself listed once
+// (the pairs feed unions, for which a repeat is harmless).  One atomic per
+// wave on the frame's link count.
 __global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   if (f.prev_mask == nullptr)
@@ -647,6 +651,32 @@ __global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ 
   const SF_GLOBAL run4 *runs = run_list(f);
   const SF_GLOBAL uint32_t *comp_of = gmem(f.comp_of);
   const SF_GLOBAL uint32_t *prev_comp = gmem(f.prev_comp_of);
+  // The components found along the run, consecutive repeats dropped, from
+  // its columns of the previous frame's bitmask row; `emit(there)` is called
+  // for each.
+  auto walk = [&](const run4 &r, auto emit) {
+    const size_t off = static_cast<size_t>(r.z) * f.wpr;
+    const SF_GLOBAL uint32_t *row = gmem(f.prev_mask) + off;
+    const SF_GLOBAL uint32_t *pre = gmem(f.prev_prefix) + off;
+    const uint32_t x1 = r.y + r.w;
+    uint32_t carry = 0, last = 0xFFFFFFFFu;
+    for (uint32_t j = r.y >> 5; j <= (x1 - 1u) >> 5; j++) {
+      const uint32_t full = row[j];
+      const uint32_t base = pre[j]; // read beside the word, used if it is set
+      const uint32_t a = full & columns(j, r.y, x1);
+      uint32_t segs = a & ~((a << 1) | carry);
+      carry = a >> 31;
+      while (segs) {
+        const uint32_t b = static_cast<uint32_t>(__ffs(segs)) - 1u;
+        segs &= segs - 1u;
+        const uint32_t q = base + static_cast<uint32_t>(__popc(full & ((1u << b) - 1u)));
+        const uint32_t there = q < prev_n ? prev_comp[q] : last;
+        if (there != last)
+          emit(there);
+        last = there;
+      }
+    }
+  };
   // Wave-uniform steps over the runs, so that whole waves meet at the scan.
   for (uint32_t step = blockIdx.x * blockDim.x + (threadIdx.x & ~(kLanes - 1u));
        step < nr; step += gridDim.x * blockDim.x) {
@@ -655,31 +685,22 @@ __global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ 
     run4 r = {0u, 0u, 0u, 0u};
     if (active)
       r = runs[k];
-    const size_t off = static_cast<size_t>(r.z) * f.wpr;
-    const SF_GLOBAL uint32_t *row = gmem(f.prev_mask) + off;
-    const SF_GLOBAL uint32_t *pre = gmem(f.prev_prefix) + off;
-    const uint32_t x1 = r.y + r.w;
-    // Two passes over the run's columns: count the distinct consecutive
-    // components, then write them at the wave's place in the list.
+    // One walk finds the run's links and keeps the first kLinksHeld in
+    // registers; only a run with more walks again to write them.
     uint32_t here = 0, count = 0;
+    uint32_t held[kLinksHeld];
+#pragma unroll
+    for (uint32_t i = 0; i < kLinksHeld; i++)
+      held[i] = 0;
     if (active) {
       here = comp_of[r.x];
-      uint32_t carry = 0, last = 0xFFFFFFFFu;
-      for (uint32_t j = r.y >> 5; j <= (x1 - 1u) >> 5; j++) {
-        const uint32_t full = row[j];
-        const uint32_t a = full & columns(j, r.y, x1);
-        uint32_t segs = a & ~((a << 1) | carry);
-        carry = a >> 31;
-        while (segs) {
-          const uint32_t b = static_cast<uint32_t>(__ffs(segs)) - 1u;
-          segs &= segs - 1u;
-          const uint32_t q =
-              pre[j] + static_cast<uint32_t>(__popc(full & ((1u << b) - 1u)));
-          const uint32_t there = q < prev_n ? prev_comp[q] : last;
-          count += there != last ? 1u : 0u;
-          last = there;
-        }
-      }
+      walk(r, [&](uint32_t there) {
+#pragma unroll
+        for (uint32_t i = 0; i < kLinksHeld; i++)
+          if (count == i)
+            held[i] = there;
+        count++;
+      });
     }
     uint32_t total;
     uint32_t at = wave_prefix(count, lane, &total);
@@ -690,29 +711,20 @@ __global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ 
       base = __hip_atomic_fetch_add(gmem(f.counter) + 3, total, __ATOMIC_RELAXED,
                                     __HIP_MEMORY_SCOPE_AGENT);
     at += __shfl(base, 0);
-    if (count) {
-      uint32_t carry = 0, last = 0xFFFFFFFFu;
-      for (uint32_t j = r.y >> 5; j <= (x1 - 1u) >> 5; j++) {
-        const uint32_t full = row[j];
-        const uint32_t a = full & columns(j, r.y, x1);
-        uint32_t segs = a & ~((a << 1) | carry);
-        carry = a >> 31;
-        while (segs) {
-          const uint32_t b = static_cast<uint32_t>(__ffs(segs)) - 1u;
-          segs &= segs - 1u;
-          const uint32_t q =
-              pre[j] + static_cast<uint32_t>(__popc(full & ((1u << b) - 1u)));
-          const uint32_t there = q < prev_n ? prev_comp[q] : last;
-          if (there != last) {
-            if (at < f.link_capacity) {
-              gmem(f.links)[2 * at] = here;
-              gmem(f.links)[2 * at + 1] = there;
-            }
-            at++;
-          }
-          last = there;
-        }
+    auto put = [&](uint32_t there) {
+      if (at < f.link_capacity) {
+        gmem(f.links)[2 * at] = here;
+        gmem(f.links)[2 * at + 1] = there;
       }
+      at++;
+    };
+    if (count <= kLinksHeld) {
+#pragma unroll
+      for (uint32_t i = 0; i < kLinksHeld; i++)
+        if (i < count)
+          put(held[i]);
+    } else {
+      walk(r, put);
     }
   }
 }
