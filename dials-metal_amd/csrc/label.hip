@@ -118,8 +118,9 @@ __device__ __forceinline__ uint32_t wave_prefix(uint32_t v, uint32_t lane,
 }
 
 // Bitmask words a lane holds at once in count_runs and emit_runs: a row of
-// up to 64 x kRowChunks words (16384 columns) in one round of loads.
-constexpr uint32_t kRowChunks = 4;
+// up to 64 x kRowChunks words (6144 columns) in one round of loads, wider
+// rows in several.
+constexpr uint32_t kRowChunks = 3;
 
 // Lane shifts across the whole wave (DPP wave_shr:1 / wave_shl:1 with
 // bound_ctrl): lane i gets lane i-1's value (lane 0 gets 0), or lane i+1's

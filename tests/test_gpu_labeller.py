@@ -46,6 +46,49 @@ def test_device_labels_equal_the_model(device, sweep):
     ws.close()
 
 
+def wide_signal(rng, h, w):
+    """An ascending signal list over an h x w frame whose rows hold runs of
+    every length from 1 to a few hundred pixels, some ending on and some
+    crossing 32-pixel words, 2048-pixel chunks and 6144-pixel rounds of the
+    device's row passes, and runs that touch the rows above and below."""
+    mask = np.zeros((h, w), bool)
+    for y in range(h):
+        x = int(rng.integers(0, 40))
+        while x < w:
+            n = int(rng.choice([1, 2, 5, 31, 32, 33, 64, 97, 300]))
+            mask[y, x:x + n] = True
+            x += n + int(rng.integers(1, 400))
+        for edge in (2047, 2048, 6143, 6144, 12287):
+            if edge < w and rng.uniform() < 0.5:
+                mask[y, max(0, edge - 20):edge + 20] = True
+    idx = np.flatnonzero(mask).astype(np.uint32)
+    sig = np.zeros(len(idx), O.SIGNAL_PIXEL)
+    sig["index"] = idx
+    sig["value"] = rng.integers(1, 1 << 20, len(idx))
+    return sig
+
+
+def test_device_labels_equal_the_model_on_wide_rows(device):
+    rng = np.random.default_rng(77)
+    h, w = 24, 13000
+    ws = device.Workspace()
+    prev = None
+    for z in range(3):
+        sig = wide_signal(rng, h, w)
+        say("wide rows", z, len(sig))
+        device.workspace_load_signal(ws, sig, h, w)
+        comp_of, comps, links, linked = device.workspace_label(ws, True)
+        m_comp_of, m_comps, m_rank = label_frame(sig, w)
+        assert (comp_of == m_comp_of).all()
+        assert comps.tobytes() == m_comps.tobytes()
+        if prev is not None:
+            want = links_between(sig, m_comp_of, prev[0], prev[1])
+            got = np.unique(links, axis=0) if len(links) else links
+            assert got.tolist() == want.tolist()
+        prev = (m_rank, m_comp_of)
+    ws.close()
+
+
 def test_device_labeller_cases(device):
     run_cases(lambda h, w, **kw: device.Labeller(h, w, **kw),
               device.LabellerError)
