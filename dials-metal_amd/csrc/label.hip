@@ -372,7 +372,8 @@ __global__ __launch_bounds__(256) void unite_runs(const FrameDev *__restrict__ f
 
 // Flatten every run's tree to its root and mark the roots (runs that are
 // their own root, each the first pixel of its component) in the bitmask of
-// ranks, one atomic per root.
+// ranks, one atomic per root.  Run k also clears the kth count of runs per
+// component: components never outnumber runs.
 __global__ __launch_bounds__(256) void flatten(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   const uint32_t nr = f.counter[4];
@@ -381,6 +382,7 @@ __global__ __launch_bounds__(256) void flatten(const FrameDev *__restrict__ fram
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
        k += gridDim.x * blockDim.x) {
     const uint32_t i = runs[k].x;
+    gmem(f.comp_runs)[k] = 0;
     uint32_t r = load_relaxed(&parent[i]);
     for (;;) {
       const uint32_t up = load_relaxed(&parent[r]);
@@ -402,39 +404,25 @@ __device__ __forceinline__ uint32_t component_of(const FrameDev &f, uint32_t r) 
          static_cast<uint32_t>(__popc(gmem(f.roots)[r / 32] & ((1u << (r & 31u)) - 1u)));
 }
 
-// Each root run opens its component (its run count starts at zero); the run
-// holding the last rank writes the component count.
-__global__ __launch_bounds__(256) void open_components(
+// Each run takes a place among its component's runs, one atomic per run (a
+// component's few runs contend, nothing else does); its component and place
+// wait in the link list's memory, which is free until the links are listed.
+// The run holding the last rank writes the component count.
+__global__ __launch_bounds__(256) void count_component_runs(
     const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.y];
   const uint32_t nr = f.counter[4];
   const uint32_t n = live_count(f);
   const SF_GLOBAL uint32_t *parent = gmem(f.parent);
   const SF_GLOBAL run4 *runs = run_list(f);
-  for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
-       k += gridDim.x * blockDim.x) {
-    const run4 r = runs[k];
-    if (parent[r.x] == r.x)
-      gmem(f.comp_runs)[component_of(f, r.x)] = 0;
-    if (r.x + r.w == n)
-      f.counter[2] = component_of(f, n - 1) +
-                     ((gmem(f.roots)[(n - 1) / 32] >> ((n - 1) & 31u)) & 1u);
-  }
-}
-
-// Each run takes a place among its component's runs, one atomic per run (a
-// component's few runs contend, nothing else does); its component and place
-// wait in the link list's memory, which is free until the links are listed.
-__global__ __launch_bounds__(256) void count_component_runs(
-    const FrameDev *__restrict__ frames) {
-  const FrameDev f = frames[blockIdx.y];
-  const uint32_t nr = f.counter[4];
-  const SF_GLOBAL uint32_t *parent = gmem(f.parent);
-  const SF_GLOBAL run4 *runs = run_list(f);
   SF_GLOBAL uint32_t *scratch = gmem(f.links);
   for (uint32_t k = blockIdx.x * blockDim.x + threadIdx.x; k < nr;
        k += gridDim.x * blockDim.x) {
-    const uint32_t c = component_of(f, parent[runs[k].x]);
+    const run4 r = runs[k];
+    if (r.x + r.w == n)
+      f.counter[2] = component_of(f, n - 1) +
+                     ((gmem(f.roots)[(n - 1) / 32] >> ((n - 1) & 31u)) & 1u);
+    const uint32_t c = component_of(f, parent[r.x]);
     scratch[2 * k] = c;
     scratch[2 * k + 1] = __hip_atomic_fetch_add(gmem(f.comp_runs) + c, 1u,
                                                 __ATOMIC_RELAXED,
@@ -748,7 +736,6 @@ void launch_label(const FrameDev *frames_device, const ScanSeg *root_segs_device
   unite_runs<<<grid, 256, 0, stream>>>(frames_device);
   flatten<<<grid, 256, 0, stream>>>(frames_device);
   launch_popcount_scan(root_segs_device, nframes, max_root_words, stream);
-  open_components<<<grid, 256, 0, stream>>>(frames_device);
   count_component_runs<<<grid, 256, 0, stream>>>(frames_device);
   scan_components<<<nframes, 1024, 0, stream>>>(frames_device);
   place_runs<<<grid, 256, 0, stream>>>(frames_device);
