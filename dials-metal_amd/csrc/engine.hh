@@ -18,7 +18,7 @@ namespace sfi {
 // One frame's buffers, kept between calls and grown to the largest frame.
 struct FrameSlot {
   DeviceBuffer mask, prefix, totals, unordered, ordered;
-  DeviceBuffer parent, roots, root_prefix, root_totals, comp_of, comps, links;
+  DeviceBuffer parent, roots, root_prefix, comp_of, comps, links;
   DeviceBuffer row_runs; // runs per bitmask row, then each row's first run
   DeviceBuffer comp_runs; // runs per component, then each component's first
   uint32_t height = 0, width = 0, wpr = 0, capacity = 0;
@@ -101,7 +101,6 @@ private:
   void shape_label(FrameSlot &s);
   FrameDev dev(int id, int prev);
   ScanSeg mask_seg(int id);
-  ScanSeg root_seg(int id);
   FrameArgs args(int id, const void *frame);
   // Pinned host staging for descriptors, two buffers used in turn, each
   // reused only once the copy that last read it has run.

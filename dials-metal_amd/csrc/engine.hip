@@ -157,7 +157,6 @@ void Engine::shape_label(FrameSlot &s) {
   grow(s.parent, (cap + 1) * 4);
   grow(s.roots, s.root_words * 4);
   grow(s.root_prefix, s.root_words * 4);
-  grow(s.root_totals, scan_tmp_words(s.root_words) * 4);
   grow(s.row_runs, (static_cast<size_t>(s.height) + 1) * 4);
   grow(s.comp_runs, (cap + 2) * 4);
   grow(s.comp_of, (cap + 1) * 4);
@@ -179,6 +178,7 @@ FrameDev Engine::dev(int id, int prev) {
   f.wpr = s.wpr;
   f.mask = s.mask.as<uint32_t>();
   f.prefix = s.prefix.as<uint32_t>();
+  f.totals = s.totals.as<uint32_t>();
   f.parent = s.parent.as<uint32_t>();
   f.roots = s.roots.as<uint32_t>();
   f.root_prefix = s.root_prefix.as<uint32_t>();
@@ -205,13 +205,6 @@ ScanSeg Engine::mask_seg(int id) {
                  s.totals.as<uint32_t>(), s.words, nullptr, 0, 1};
 }
 
-ScanSeg Engine::root_seg(int id) {
-  FrameSlot &s = slot(id);
-  return ScanSeg{s.roots.as<uint32_t>(), s.root_prefix.as<uint32_t>(),
-                 s.root_totals.as<uint32_t>(), s.root_words,
-                 counters_.as<uint32_t>() + kCounters * id, s.capacity, 0};
-}
-
 FrameArgs Engine::args(int id, const void *frame) {
   FrameSlot &s = slot(id);
   FrameArgs a;
@@ -228,6 +221,7 @@ FrameArgs Engine::args(int id, const void *frame) {
                 ? table_.ptr
                 : static_cast<const void *>(table_.as<char>() + 128 * sizeof(float));
   a.overflow = a.counter + 1;
+  a.totals = s.totals.as<uint32_t>();
   return a;
 }
 
@@ -298,7 +292,8 @@ void Engine::enqueue_kernels(const Prepared &p, const FrameIn *frames,
   }
   if (timing)
     check(hipEventRecord(marks_[1], stream_), "hipEventRecord");
-  launch_popcount_scan(sg_d, nf, p.max_words, stream_);
+  // The fused kernel counts each block's signal pixels as it emits them.
+  launch_popcount_scan(sg_d, nf, p.max_words, stream_, fused);
   launch_scatter(fd_d, nf, stream_);
   if (timing)
     check(hipEventRecord(marks_[2], stream_), "hipEventRecord");
@@ -475,27 +470,19 @@ void Engine::label(const int *ids, const int *prev, size_t n) {
     return;
   for (size_t i = 0; i < n; i++)
     shape_label(slot(ids[i]));
-  const size_t b_frames = align16(n * sizeof(FrameDev));
-  const size_t bytes = b_frames + align16(n * sizeof(ScanSeg));
+  const size_t bytes = align16(n * sizeof(FrameDev));
   if (bytes > kDescHalf)
     throw device_error("too many frames in one batch");
   char *host = stage(bytes);
   FrameDev *fd = reinterpret_cast<FrameDev *>(host);
-  ScanSeg *rs = reinterpret_cast<ScanSeg *>(host + b_frames);
-  size_t max_root_words = 0;
-  for (size_t i = 0; i < n; i++) {
+  for (size_t i = 0; i < n; i++)
     fd[i] = dev(ids[i], prev ? prev[i] : -1);
-    rs[i] = root_seg(ids[i]);
-    max_root_words = rs[i].words > max_root_words ? rs[i].words : max_root_words;
-  }
   char *d = desc_.as<char>() + kDescHalf;
   check(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream_),
         "uploading label descriptors");
   staged();
-  const FrameDev *fd_d = reinterpret_cast<const FrameDev *>(d);
-  const ScanSeg *rs_d = reinterpret_cast<const ScanSeg *>(d + b_frames);
-  launch_clear(fd_d, static_cast<uint32_t>(n), false, 2, stream_);
-  launch_label(fd_d, rs_d, static_cast<uint32_t>(n), max_root_words, stream_);
+  launch_label(reinterpret_cast<const FrameDev *>(d), static_cast<uint32_t>(n),
+               stream_);
   for (size_t i = 0; i < n; i++) {
     slot(ids[i]).labelled = true;
     slot(ids[i]).linked_serial =

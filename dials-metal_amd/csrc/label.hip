@@ -195,6 +195,11 @@ __global__ __launch_bounds__(256) void count_runs(const FrameDev *__restrict__ f
 __global__ __launch_bounds__(1024) void scan_runs(const FrameDev *__restrict__ frames) {
   const FrameDev f = frames[blockIdx.x];
   __shared__ uint32_t wave_sums[1024 / kLanes];
+  // The components are counted later; the links accumulate from zero.
+  if (threadIdx.x == 0) {
+    f.counter[2] = 0;
+    f.counter[3] = 0;
+  }
   if (live_count(f) == 0) {
     if (threadIdx.x == 0)
       f.counter[4] = 0;
@@ -395,6 +400,46 @@ __global__ __launch_bounds__(256) void flatten(const FrameDev *__restrict__ fram
                             __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     else
       store_relaxed(&parent[i], r);
+  }
+}
+
+// Component numbers: an exclusive popcount scan of each frame's root bitmask
+// over its live words, one block of 1024 threads a frame, each thread four
+// consecutive words a round.
+__global__ __launch_bounds__(1024) void number_roots(const FrameDev *__restrict__ frames) {
+  __shared__ uint32_t wave_sums[1024 / kLanes];
+  const FrameDev f = frames[blockIdx.x];
+  const uint32_t words = (live_count(f) + 31u) / 32u;
+  const SF_GLOBAL uint32_t *roots = gmem(f.roots);
+  SF_GLOBAL uint32_t *prefix = gmem(f.root_prefix);
+  const uint32_t lane = threadIdx.x % kLanes, wave = threadIdx.x / kLanes;
+  uint32_t carry = 0;
+  for (uint32_t w0 = 0; w0 < words; w0 += 4u * blockDim.x) {
+    const uint32_t w = w0 + 4u * threadIdx.x;
+    uint32_t c[4];
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++)
+      c[k] = w + k < words ? static_cast<uint32_t>(__popc(roots[w + k])) : 0u;
+    const uint32_t mine = (c[0] + c[1]) + (c[2] + c[3]);
+    uint32_t total;
+    uint32_t run = wave_prefix(mine, lane, &total);
+    if (lane == 0)
+      wave_sums[wave] = total;
+    __syncthreads();
+    uint32_t all = 0;
+    for (uint32_t v = 0; v < blockDim.x / kLanes; v++) {
+      run += v < wave ? wave_sums[v] : 0u;
+      all += wave_sums[v];
+    }
+    __syncthreads();
+    run += carry;
+#pragma unroll
+    for (uint32_t k = 0; k < 4; k++) {
+      if (w + k < words)
+        prefix[w + k] = run;
+      run += c[k];
+    }
+    carry += all;
   }
 }
 
@@ -722,11 +767,9 @@ __global__ __launch_bounds__(256) void link_frames(const FrameDev *__restrict__ 
 
 namespace sfi {
 
-// frames_device: the frames' descriptors; root_segs_device: their root-bit
-// scans; both in device memory, nframes long.  counter[2], counter[3] and
-// counter[4] of every frame are cleared by the caller.
-void launch_label(const FrameDev *frames_device, const ScanSeg *root_segs_device,
-                  uint32_t nframes, size_t max_root_words, hipStream_t stream) {
+// frames_device: the frames' descriptors in device memory, nframes long.
+void launch_label(const FrameDev *frames_device, uint32_t nframes,
+                  hipStream_t stream) {
   if (nframes == 0)
     return;
   const dim3 grid(256, nframes);
@@ -735,7 +778,7 @@ void launch_label(const FrameDev *frames_device, const ScanSeg *root_segs_device
   emit_runs<<<grid, 256, 0, stream>>>(frames_device);
   unite_runs<<<grid, 256, 0, stream>>>(frames_device);
   flatten<<<grid, 256, 0, stream>>>(frames_device);
-  launch_popcount_scan(root_segs_device, nframes, max_root_words, stream);
+  number_roots<<<nframes, 1024, 0, stream>>>(frames_device);
   count_component_runs<<<grid, 256, 0, stream>>>(frames_device);
   scan_components<<<nframes, 1024, 0, stream>>>(frames_device);
   place_runs<<<grid, 256, 0, stream>>>(frames_device);

@@ -215,6 +215,10 @@ __global__ __launch_bounds__(256) void clear_frames(
   for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
        i < words; i += static_cast<size_t>(gridDim.x) * blockDim.x)
     f.mask[i] = 0;
+  const size_t blocks = (words + kScanPerBlock - 1) / kScanPerBlock;
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+       i < blocks; i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    f.totals[i] = 0;
 }
 
 __global__ __launch_bounds__(256) void fill_words(uint32_t *p, size_t n,
@@ -233,12 +237,14 @@ size_t scan_tmp_words(size_t words) {
 }
 
 void launch_popcount_scan(const ScanSeg *segs_device, uint32_t nframes,
-                          size_t max_words, hipStream_t stream) {
+                          size_t max_words, hipStream_t stream,
+                          bool totals_ready) {
   const uint32_t blocks =
       static_cast<uint32_t>((max_words + kScanPerBlock - 1) / kScanPerBlock);
   if (blocks == 0 || nframes == 0)
     return;
-  popcount_totals<<<dim3(blocks, nframes), kScanThreads, 0, stream>>>(segs_device);
+  if (!totals_ready)
+    popcount_totals<<<dim3(blocks, nframes), kScanThreads, 0, stream>>>(segs_device);
   scan_totals<<<dim3(1, nframes), kScanThreads, 0, stream>>>(segs_device);
   popcount_prefix<<<dim3(blocks, nframes), kScanThreads, 0, stream>>>(segs_device);
   check(hipGetLastError(), "launching a popcount scan");

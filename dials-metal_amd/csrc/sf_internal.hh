@@ -94,6 +94,8 @@ struct FrameArgs {
   const void *table;      // R(sqrt(2 (m - 1))) for m = 0..127, R per bits
   uint32_t *overflow;     // device word, set by the fused kernel when a
                           // valid pixel is too large for its fast sums
+  uint32_t *totals;       // the bitmask's set bits per kScanPerBlock words,
+                          // counted by the fused kernel (cleared before it)
 };
 
 // The three direct-window kernels (the cross-check form): scr0 and scr1 are
@@ -114,6 +116,7 @@ struct FusedParams {
   const void *table;
   uint32_t strips, segments, seg_rows;
   uint32_t *overflow;
+  uint32_t *totals;
   uint32_t bits;
   uint32_t item_base;
 };
@@ -161,9 +164,11 @@ struct ScanSeg {
   uint32_t sparse;
 };
 size_t scan_tmp_words(size_t words);
-// One scan per frame, all frames in three launches; descriptors on device.
+// One scan per frame, all frames in three launches (two when the block
+// totals are already counted); descriptors on device.
 void launch_popcount_scan(const ScanSeg *segs_device, uint32_t nframes,
-                          size_t max_words, hipStream_t stream);
+                          size_t max_words, hipStream_t stream,
+                          bool totals_ready = false);
 
 // Words of counters per frame: [0] signal pixels [1] the fused kernel's
 // overflow flag [2] components [3] frame links [4] row runs.
@@ -180,6 +185,7 @@ struct FrameDev {
   uint32_t capacity;
   uint32_t width, height, wpr;
   uint32_t *mask, *prefix;    // strong-pixel bitmask and its word prefix
+  uint32_t *totals;           // the bitmask's set bits per kScanPerBlock words
   // labelling
   uint32_t *parent, *roots, *root_prefix, *comp_of;
   uint32_t *row_runs; // height words: runs per row, then each row's first run
@@ -197,7 +203,8 @@ struct FrameDev {
 };
 
 // Clears each frame's counters from `first` to the last and, with `masks`,
-// its bitmask; zeroes `queue` when given (the fused kernel's item queue).
+// its bitmask and block totals; zeroes `queue` when given (the fused
+// kernel's item queue).
 void launch_clear(const FrameDev *frames_device, uint32_t nframes, bool masks,
                   int first, hipStream_t stream, uint32_t *queue = nullptr);
 // Places every frame's unordered emit at its rank (blockIdx.y = frame).
@@ -208,7 +215,7 @@ void launch_scatter(const FrameDev *frames_device, uint32_t nframes,
 void launch_set_bits(const FrameDev *frames_device, uint32_t nframes,
                      hipStream_t stream);
 // Labels every frame: components, moments, and links where prev_mask is set.
-void launch_label(const FrameDev *frames_device, const ScanSeg *root_segs_device,
-                  uint32_t nframes, size_t max_root_words, hipStream_t stream);
+void launch_label(const FrameDev *frames_device, uint32_t nframes,
+                  hipStream_t stream);
 
 } // namespace sfi

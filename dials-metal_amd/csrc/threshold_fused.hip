@@ -352,6 +352,21 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
         __hip_atomic_fetch_or(mask + mword, 1u << mbit, __ATOMIC_RELAXED,
                               __HIP_MEMORY_SCOPE_AGENT);
       }
+      // The ordering scan's block totals: the bits set in each block of
+      // kScanPerBlock mask words, one atomic per block the flush touches
+      // (its pixels lie in a few rows, so one or two blocks).
+      const uint32_t block = mword / static_cast<uint32_t>(sfi::kScanPerBlock);
+      for (uint64_t todo = b; todo != 0;) {
+        const uint32_t first = static_cast<uint32_t>(__builtin_ctzll(todo));
+        const uint32_t fb = static_cast<uint32_t>(__builtin_amdgcn_readlane(
+            static_cast<int>(block), static_cast<int>(first)));
+        const uint64_t same = __ballot(signal && block == fb) & todo;
+        if (lane == first)
+          __hip_atomic_fetch_add(gmem(a.totals) + fb,
+                                 static_cast<uint32_t>(__popcll(same)),
+                                 __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        todo &= ~same;
+      }
     }
     __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
     __builtin_amdgcn_wave_barrier();
@@ -896,6 +911,7 @@ FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,
   p.segments = plan.segments;
   p.seg_rows = plan.seg_rows;
   p.overflow = a.overflow;
+  p.totals = a.totals;
   p.bits = static_cast<uint32_t>(a.bits);
   p.item_base = item_base;
   return p;
