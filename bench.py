@@ -340,10 +340,9 @@ def main():
         t0 = time.perf_counter()
         for d in dev:
             ws.threshold(d.ptr, 32, h, w, "fused")
-            rc, _ = ws.sync()
             if label:
                 sf.workspace_label_enqueue(ws)
-                ws.sync()
+            rc, _ = ws.sync()
         plain = len(dev) / (time.perf_counter() - t0)
         ws.close()
         line["plain_per_frame"] = {"value": plain, "unit": "frames/s",

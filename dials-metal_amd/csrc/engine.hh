@@ -8,6 +8,7 @@
 // the previous one is read.
 #pragma once
 
+#include <map>
 #include <memory>
 #include <vector>
 
@@ -133,6 +134,15 @@ private:
   hipGraph_t graph_ = nullptr;
   hipGraphExec_t graph_exec_ = nullptr;
   std::vector<uint64_t> graph_key_;
+
+  // The labelling sequence, captured once per frame count: its kernels take
+  // only the descriptor block, at a fixed address, and the frame count.
+  struct LabelGraph {
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+  };
+  std::map<uint32_t, LabelGraph> label_graphs_;
+  void drop_label_graphs();
 };
 
 } // namespace sfi

@@ -15,6 +15,7 @@ size_t align16(size_t b) { return (b + 15) & ~static_cast<size_t>(15); }
 // The labelling's descriptors live in the second half of desc_, the
 // threshold's in the first, so a captured threshold keeps its own.
 constexpr size_t kDescHalf = 1 << 20;
+constexpr size_t kLabelGraphs = 16; // captured frame counts kept at once
 
 } // namespace
 
@@ -57,6 +58,7 @@ Engine::~Engine() {
     (void)hipGraphExecDestroy(graph_exec_);
   if (graph_)
     (void)hipGraphDestroy(graph_);
+  drop_label_graphs();
   for (hipEvent_t &e : marks_)
     if (e)
       (void)hipEventDestroy(e);
@@ -481,14 +483,43 @@ void Engine::label(const int *ids, const int *prev, size_t n) {
   check(hipMemcpyAsync(d, host, bytes, hipMemcpyHostToDevice, stream_),
         "uploading label descriptors");
   staged();
-  launch_label(reinterpret_cast<const FrameDev *>(d), static_cast<uint32_t>(n),
-               stream_);
+  // Every buffer the labelling touches is named by the descriptors just
+  // uploaded, so one capture per frame count serves any frames and slots.
+  const uint32_t nframes = static_cast<uint32_t>(n);
+  auto it = label_graphs_.find(nframes);
+  if (it == label_graphs_.end()) {
+    if (label_graphs_.size() >= kLabelGraphs)
+      drop_label_graphs();
+    LabelGraph g;
+    check(hipStreamBeginCapture(stream_, hipStreamCaptureModeThreadLocal),
+          "hipStreamBeginCapture");
+    launch_label(reinterpret_cast<const FrameDev *>(d), nframes, stream_);
+    check(hipStreamEndCapture(stream_, &g.graph), "hipStreamEndCapture");
+    const hipError_t e =
+        hipGraphInstantiate(&g.exec, g.graph, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+      (void)hipGraphDestroy(g.graph);
+      check(e, "hipGraphInstantiate");
+    }
+    it = label_graphs_.emplace(nframes, g).first;
+  }
+  check(hipGraphLaunch(it->second.exec, stream_), "hipGraphLaunch");
   for (size_t i = 0; i < n; i++) {
     slot(ids[i]).labelled = true;
     slot(ids[i]).linked_serial =
         (prev && prev[i] >= 0) ? slot(prev[i]).serial : 0;
     slot(ids[i]).linked_slot = (prev && prev[i] >= 0) ? prev[i] : -1;
   }
+}
+
+void Engine::drop_label_graphs() {
+  for (auto &entry : label_graphs_) {
+    if (entry.second.exec)
+      (void)hipGraphExecDestroy(entry.second.exec);
+    if (entry.second.graph)
+      (void)hipGraphDestroy(entry.second.graph);
+  }
+  label_graphs_.clear();
 }
 
 void Engine::label_counts(int id, size_t *n, size_t *ncomp, size_t *nlinks,

@@ -193,7 +193,8 @@ void ws_threshold(sf_workspace *ws, int bits, const void *frame, uint32_t h,
                   uint32_t w, int kernel) {
   ws->cur ^= 1;
   const FrameIn f{frame, h, w, bits};
-  ws->engine.threshold(&f, &ws->cur, 1, kernel, false);
+  // The fused sequence replays from one capture per frame shape.
+  ws->engine.threshold(&f, &ws->cur, 1, kernel, kernel == SF_KERNEL_FUSED);
 }
 
 int ws_sync(sf_workspace *ws, size_t *found) {

@@ -67,6 +67,7 @@ using namespace sfd;
 
 constexpr int kLanes = 64;
 constexpr int kHalo = 10;               // columns of halo each side: 3 + 2 + 5
+constexpr int kAhead = 2;               // rows a wave keeps in flight from HBM
 constexpr int kRing = 16;               // raw rows held: entering .. 15 above
 constexpr uint32_t kStage = 48;         // candidates staged per wave
 constexpr uint32_t kFast = 1u << 18;    // fast-path bound on valid pixels
@@ -439,13 +440,14 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
   uint32_t wide_rows = 0, x1_rows = 0;
 
   const int steps = (y1 - y0) + 20;
-  // Two rows in flight from HBM: the buffer a row is read from takes the load
-  // of the row two below as soon as its values are in registers.
-  uint32_t ahead0[KQ] = {}, ahead1[KQ] = {};
-  if (y0 - 10 >= 0 && y0 - 10 < height)
-    load_row<T, KQ>(frame, width, y0 - 10, c0, ahead0);
-  if (y0 - 9 >= 0 && y0 - 9 < height)
-    load_row<T, KQ>(frame, width, y0 - 9, c0, ahead1);
+  // kAhead rows in flight from HBM: the buffer a row is read from takes the
+  // load of the row kAhead below as soon as its values are in registers.
+  // (Four in flight measured no faster, alone on the device or batched.)
+  uint32_t ahead[kAhead][KQ] = {};
+#pragma unroll
+  for (int k = 0; k < kAhead; k++)
+    if (y0 - 10 + k >= 0 && y0 - 10 + k < height)
+      load_row<T, KQ>(frame, width, y0 - 10 + k, c0, ahead[k]);
   int slot = 0; // ring slot of the entering row
   auto row = [&](int t, uint32_t(&buf)[KQ]) {
     const int y_in = y0 - 10 + t;
@@ -482,8 +484,8 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
         wide = __ballot(any >= kWide) != 0;
       }
     }
-    if (t + 2 < steps && y_in + 2 >= 0 && y_in + 2 < height)
-      load_row<T, KQ>(frame, width, y_in + 2, c0, buf);
+    if (t + kAhead < steps && y_in + kAhead >= 0 && y_in + kAhead < height)
+      load_row<T, KQ>(frame, width, y_in + kAhead, c0, buf);
 
     // Ring rows leaving stage 0 (y_in - 7), entering stage 2 (y_in - 5) and
     // leaving it (y_in - 16, whose slot the entering row takes once it has
@@ -745,13 +747,12 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     }
     slot = (slot + 1) & (kRing - 1);
   };
-  int t = 0;
-  for (; t + 1 < steps; t += 2) {
-    row(t, ahead0);
-    row(t + 1, ahead1);
+  for (int t = 0; t < steps; t += kAhead) {
+#pragma unroll
+    for (int k = 0; k < kAhead; k++)
+      if (k == 0 || t + k < steps)
+        row(t + k, ahead[k]);
   }
-  if (t < steps)
-    row(t, ahead0);
   flush();
   if (__ballot(big) != 0 && lane == 0)
     __hip_atomic_fetch_or(gmem(a.overflow), 1u, __ATOMIC_RELAXED,
