@@ -132,6 +132,14 @@ class SynthDesc(ctypes.Structure):
                 ("persist", ctypes.c_uint32), ("threads", ctypes.c_uint32)]
 
 
+class Chunk(ctypes.Structure):
+    """sf_chunk: a chunk as HDF5 stored it, and its frame in device memory."""
+    _fields_ = [("data", ctypes.c_void_p), ("size", ctypes.c_size_t),
+                ("algorithm", ctypes.c_char_p), ("bit_depth", ctypes.c_uint32),
+                ("height", ctypes.c_uint32), ("width", ctypes.c_uint32),
+                ("frame", ctypes.c_void_p)]
+
+
 _lib = None
 
 
@@ -237,6 +245,17 @@ def lib():
         "sf_synth_default": ([ctypes.POINTER(SynthDesc), u32, u32], None),
         "sf_synth_eiger": ([ctypes.POINTER(SynthDesc), ctypes.c_int, ctypes.c_int], None),
         "sf_synth_frame": ([ctypes.POINTER(SynthDesc), ctypes.c_int64, vp], ctypes.c_int),
+        "sf_frame_bytes": ([sz, sz, ctypes.c_uint], sz),
+        "sf_decompress_image": ([vp, sz, ctypes.c_char_p, ctypes.c_uint, u32, u32,
+                                 vp, sz], ctypes.c_int),
+        "sf_decoder_create": ([vp], vp),
+        "sf_decoder_destroy": ([vp], None),
+        "sf_decoder_stream": ([vp], vp),
+        "sf_decoder_load": ([vp, ctypes.POINTER(Chunk), sz], ctypes.c_int),
+        "sf_decoder_run": ([vp], ctypes.c_int),
+        "sf_decoder_sync": ([vp, ctypes.POINTER(ctypes.c_int)], ctypes.c_int),
+        "sf_decoder_set_timing": ([vp, ctypes.c_int], None),
+        "sf_decoder_kernel_ms": ([vp, ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
     }
     for name, (args, res) in sigs.items():
         f = getattr(L, name)
@@ -731,3 +750,92 @@ def workspace_load_signal(ws, signal, height, width):
     if rc == -1:
         raise ValueError(last_error())
     _check(rc)
+
+
+# ---- decompression on the device (src/spots/decompress.hh:22-30) ----
+
+class DecompressError(RuntimeError):
+    """A chunk that does not describe its frame, or malformed data: the
+    reference's decompress::image throws std::runtime_error for these."""
+
+
+def frame_bytes(height, width, bit_depth):
+    """decompress::frame_bytes (decompress.hh:18-20)."""
+    return int(lib().sf_frame_bytes(height, width, bit_depth))
+
+
+def decompress_image(data, algorithm, bit_depth, height, width, frame):
+    """decompress::image with the frame left in device memory: ``frame`` is a
+    DeviceArray (or a device address and its byte count as a tuple)."""
+    ptr, nbytes = (frame.ptr, frame.nbytes) if hasattr(frame, "ptr") else frame
+    data = bytes(data)
+    rc = lib().sf_decompress_image(data, len(data), (algorithm or "").encode(),
+                                   bit_depth, height, width, ptr, nbytes)
+    if rc == -1:
+        raise DecompressError(last_error())
+    _check(rc)
+
+
+class Decoder:
+    """sf_decoder: many chunks decoded at once.  load() checks every chunk and
+    uploads them; run() decodes them into their frames (again and again, the
+    chunks staying resident); sync() waits and returns each chunk's status."""
+
+    def __init__(self, stream=None):
+        self.h = lib().sf_decoder_create(stream)
+        if not self.h:
+            raise RuntimeError(last_error())
+        self._keep = []
+
+    def close(self):
+        if self.h:
+            lib().sf_decoder_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def stream(self):
+        return lib().sf_decoder_stream(self.h)
+
+    def load(self, chunks):
+        """chunks: (data bytes, algorithm, bit_depth, height, width, frame)
+        tuples, frame a DeviceArray or a device address."""
+        arr = (Chunk * len(chunks))()
+        keep = []
+        for i, (data, algo, bd, h, w, frame) in enumerate(chunks):
+            buf = ctypes.create_string_buffer(bytes(data), len(data))
+            keep.append(buf)
+            arr[i].data = ctypes.cast(buf, ctypes.c_void_p)
+            arr[i].size = len(data)
+            arr[i].algorithm = (algo or "").encode()
+            arr[i].bit_depth, arr[i].height, arr[i].width = bd, h, w
+            arr[i].frame = frame.ptr if hasattr(frame, "ptr") else frame
+        rc = lib().sf_decoder_load(self.h, arr, len(chunks))
+        if rc == -1:
+            raise DecompressError(last_error())
+        _check(rc)
+        self.n = len(chunks)
+
+    def run(self):
+        _check(lib().sf_decoder_run(self.h))
+
+    def sync(self):
+        """Each chunk's status: 0, or -1 where its data were malformed;
+        last_error() holds the first failure's message."""
+        st = (ctypes.c_int * max(self.n, 1))()
+        rc = lib().sf_decoder_sync(self.h, st)
+        if rc not in (0, -1):
+            _check(rc)
+        return list(st)[: self.n]
+
+    def set_timing(self, on):
+        lib().sf_decoder_set_timing(self.h, int(bool(on)))
+
+    def kernel_ms(self):
+        ms = ctypes.c_double(0)
+        _check(lib().sf_decoder_kernel_ms(self.h, ctypes.byref(ms)))
+        return ms.value

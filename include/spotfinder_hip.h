@@ -386,6 +386,57 @@ int sf_server_replay(sf_server *s, const int *sweeps,
                      const void *const *frames, const double *arrival_s,
                      size_t n, sf_server_result *results);
 
+/* ------------------------------------------------------------------------
+ * Decompression on the device: decompress::image (src/spots/decompress.hh:
+ * 22-30, decompress.cc:76-133) with the frame left in HBM, so that only the
+ * compressed chunk crosses PCIe.  The chunk is the bytes exactly as HDF5
+ * stored them, filter header and all, in host memory; `algorithm` is what the
+ * dataset's filter pipeline names (nxmx.cc:163-179): "bslz4" (bitshuffle +
+ * LZ4), "lz4", or "" / NULL for a chunk stored uncompressed.  A chunk that
+ * does not describe its frame fails with -1 and the reference's message in
+ * sf_last_error().  The device decoder's limits: bslz4 elements of 1, 2 or 4
+ * bytes, blocks of at most 65536 bytes (bitshuffle's default is 8192), frames
+ * and chunks below 4 GiB, device frames 8-byte aligned.
+ * ------------------------------------------------------------------------ */
+
+/* decompress::frame_bytes (decompress.hh:18-20). */
+size_t sf_frame_bytes(size_t height, size_t width, unsigned bit_depth);
+
+typedef struct sf_chunk {
+  const uint8_t *data;   /* host memory, `size` bytes */
+  size_t size;
+  const char *algorithm; /* "bslz4", "lz4", "" or NULL */
+  uint32_t bit_depth;    /* 8, 16 or 32 */
+  uint32_t height, width;
+  void *frame;           /* device memory, sf_frame_bytes() long */
+} sf_chunk;
+
+/* decompress::image for one chunk, on the calling thread's decoder; returns
+ * once the frame is in device memory (device_frame_bytes long).  0, -1 (the
+ * chunk does not describe the frame, or its data are malformed), -3. */
+int sf_decompress_image(const uint8_t *data, size_t size, const char *algorithm,
+                        unsigned bit_depth, uint32_t height, uint32_t width,
+                        void *device_frame, size_t device_frame_bytes);
+
+/* A decoder for many chunks at once: its stream (the caller's, or NULL for
+ * its own), pinned staging and device buffers, grow-only.  No counterpart in
+ * the reference, which decodes a chunk at a time on a host thread
+ * (find_spots.cc:766-780).  load() checks every chunk against its frame
+ * (all or nothing), walks the bslz4 block lengths on the host and enqueues
+ * one upload; run() enqueues the decoding of what the last load holds (it may
+ * be run again: the chunks stay resident); sync() waits and reports each
+ * chunk (status[i] 0 or -1; the first failure's message in sf_last_error()). */
+typedef struct sf_decoder sf_decoder;
+sf_decoder *sf_decoder_create(void *stream);
+void sf_decoder_destroy(sf_decoder *d);
+void *sf_decoder_stream(sf_decoder *d);
+int sf_decoder_load(sf_decoder *d, const sf_chunk *chunks, size_t n);
+int sf_decoder_run(sf_decoder *d);
+int sf_decoder_sync(sf_decoder *d, int *status);
+/* With timing on, the device time of the last run (its kernels and copies). */
+void sf_decoder_set_timing(sf_decoder *d, int on);
+int sf_decoder_kernel_ms(sf_decoder *d, double *ms);
+
 #ifdef __cplusplus
 }
 #endif

@@ -7,6 +7,10 @@
  *   oracle_dext_u16 / oracle_dext_u32  src/spots/dext.cc:8-225 (dext<T>)
  *   oracle_sort_by_index              src/spots/signal_order.cc:19-72
  *   oracle_labeller_*                 src/spots/dials_spots.cc:36-360
+ *   oracle_decompress_image etc.      src/spots/decompress.cc:1-136, with the
+ *                                     bitshuffle and lz4 codecs it calls
+ *                                     (oracle/decompress.c says how those are
+ *                                     restated and pinned)
  *
  * Every function here is pinned against oracle/_ref (the reference's own
  * dext.cc, signal_order.cc and dials_spots.cc compiled from their sources) and
@@ -90,6 +94,22 @@ size_t oracle_labeller_live(const oracle_labeller *l);
 uint64_t oracle_dext_u32_threads(const uint32_t *frames, size_t nframes,
                                  size_t height, size_t width, int threads,
                                  double *seconds);
+
+/* Decompression (oracle/decompress.c).  oracle_lz4_decode_block: bytes
+ * written, or < 0 for a malformed block.  oracle_bslz4_decode: bytes consumed,
+ * or bitshuffle's negative code.  oracle_decompress_image: 0, or -1 with the
+ * reference's message in err. */
+long oracle_lz4_decode_block(const uint8_t *src, size_t n, uint8_t *dst,
+                             size_t cap);
+void oracle_bitshuffle(const uint8_t *in, uint8_t *out, size_t n, size_t elem);
+void oracle_bitunshuffle(const uint8_t *in, uint8_t *out, size_t n, size_t elem);
+size_t oracle_bslz4_default_block(size_t elem);
+long oracle_bslz4_decode(const uint8_t *in, size_t n, uint8_t *out,
+                         size_t count, size_t elem, size_t block);
+int oracle_decompress_image(const uint8_t *data, size_t n, const char *algorithm,
+                            unsigned bit_depth, size_t height, size_t width,
+                            uint8_t *out, size_t out_size, char *err,
+                            size_t err_len);
 
 #ifdef __cplusplus
 }

@@ -78,6 +78,18 @@ class _Lib:
             L.oracle_labeller_pixels.restype = vp
             L.oracle_labeller_counts.argtypes = [vp]
             L.oracle_labeller_counts.restype = _Counts
+            L.oracle_lz4_decode_block.argtypes = [vp, sz, vp, sz]
+            L.oracle_lz4_decode_block.restype = ctypes.c_long
+            L.oracle_bitshuffle.argtypes = [vp, vp, sz, sz]
+            L.oracle_bitunshuffle.argtypes = [vp, vp, sz, sz]
+            L.oracle_bslz4_default_block.argtypes = [sz]
+            L.oracle_bslz4_default_block.restype = sz
+            L.oracle_bslz4_decode.argtypes = [vp, sz, vp, sz, sz, sz]
+            L.oracle_bslz4_decode.restype = ctypes.c_long
+            L.oracle_decompress_image.argtypes = [
+                vp, sz, ctypes.c_char_p, ctypes.c_uint, sz, sz, vp, sz,
+                ctypes.c_char_p, sz]
+            L.oracle_decompress_image.restype = ctypes.c_int
         else:
             L.ref_labeller_spots.argtypes = [vp, vp]
             L.ref_labeller_pixels.argtypes = [vp, vp]
@@ -113,6 +125,44 @@ class _Lib:
         sig = getattr(self.lib, f"{self.prefix}dext_u32_threads")(
             _p(frames), n, h, w, threads, ctypes.byref(secs))
         return int(sig), secs.value
+
+    # ---- decompression (oracle only: decompress.cc needs the absent
+    # bitshuffle and lz4 sources, so it has no compiled counterpart) ----
+
+    def lz4_decode_block(self, block, capacity):
+        """(status, bytes): status is the byte count, or < 0 when malformed."""
+        src = np.frombuffer(bytes(block), np.uint8)
+        out = np.zeros(max(capacity, 1), np.uint8)
+        r = self.lib.oracle_lz4_decode_block(_p(src), len(src), _p(out), capacity)
+        return r, out[: max(r, 0)].tobytes()
+
+    def bitshuffle(self, data, n, elem):
+        src = np.frombuffer(bytes(data), np.uint8)
+        out = np.zeros(n * elem, np.uint8)
+        self.lib.oracle_bitshuffle(_p(src), _p(out), n, elem)
+        return out.tobytes()
+
+    def bitunshuffle(self, data, n, elem):
+        src = np.frombuffer(bytes(data), np.uint8)
+        out = np.zeros(n * elem, np.uint8)
+        self.lib.oracle_bitunshuffle(_p(src), _p(out), n, elem)
+        return out.tobytes()
+
+    def bslz4_default_block(self, elem):
+        return int(self.lib.oracle_bslz4_default_block(elem))
+
+    def decompress_image(self, data, algorithm, bit_depth, height, width):
+        """(0, frame bytes, "") or (-1, None, the reference's message)."""
+        src = np.frombuffer(bytes(data), np.uint8)
+        expected = height * width * bit_depth // 8
+        out = np.zeros(max(expected, 1), np.uint8)
+        err = ctypes.create_string_buffer(256)
+        rc = self.lib.oracle_decompress_image(
+            _p(src) if len(src) else None, len(src), (algorithm or "").encode(),
+            bit_depth, height, width, _p(out), expected, err, len(err))
+        if rc != 0:
+            return rc, None, err.value.decode()
+        return 0, out[:expected].tobytes(), ""
 
     def labeller(self, height, width, min_spot_size=3, max_spot_size=1000,
                  max_separation=2.0, two_d=False):
