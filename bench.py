@@ -16,7 +16,10 @@ streams instead); `value` is frames per second over the timed steps.  Beside it:
 per-frame mode (one frame per launch sequence, waiting for each), the fused
 kernel's achieved HBM bandwidth against the 8 TB/s peak from HIP events on
 its stream, and the reference's own CPU threshold (compiled from its
-sources, oracle/_ref) on this host's cores over a bounded sample.
+sources, oracle/_ref) on this host's cores over a bounded sample.  The
+`decompress` object measures the step before the path: 16M frames stored as
+bslz4 chunks (the EIGER's bitshuffle-LZ4 HDF5 filter), resident in HBM and
+decoded into frames on the device.
 """
 import argparse
 import json
@@ -171,6 +174,87 @@ def uneven_batch(sf, reps, graph):
             "refused": int(sum(1 for s in status if s))}
 
 
+def decompress(sf, frames_host, distinct=4, per_step=32, steps=10):
+    """Device decompression of EIGER2 16M uint32 frames stored as bslz4 chunks
+    (bitshuffle's default 8192-byte blocks): `distinct` frames encoded on the
+    host, `per_step` chunks loaded once (resident in HBM), then each step
+    decodes all of them into their own frames.  Also the rate with the chunks
+    uploaded every step (PCIe included), and the oracle's decoder on one
+    frame on one host thread."""
+    from spotfinder_hip import codec
+    f0 = frames_host[0]
+    h, w = f0.shape
+    log(f"encoding {distinct} 16M frames as bslz4")
+    chunks = [codec.bslz4_encode(frames_host[i]) for i in range(distinct)]
+    outs = [sf.DeviceArray(f0.nbytes) for _ in range(per_step)]
+    load = [(chunks[i % distinct], "bslz4", 32, h, w, outs[i]) for i in range(per_step)]
+    dec = sf.Decoder()
+    dec.load(load)
+    dec.run()
+    if dec.sync() != [0] * per_step:
+        raise RuntimeError("a chunk failed to decode: " + sf.last_error())
+    for i in range(distinct):
+        if not np.array_equal(outs[i].to_numpy(np.uint32, (h, w)), frames_host[i]):
+            raise RuntimeError("a decoded frame differs from its original")
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        dec.run()
+        dec.sync()
+    resident = per_step * steps / (time.perf_counter() - t0)
+    dec.set_timing(True)
+    kms = []
+    for _ in range(3):
+        dec.run()
+        dec.sync()
+        kms.append(dec.kernel_ms())
+    dec.set_timing(False)
+    kms = float(np.median(kms))
+    t0 = time.perf_counter()
+    for _ in range(3):
+        dec.load(load)
+        dec.run()
+        dec.sync()
+    host = per_step * 3 / (time.perf_counter() - t0)
+    comp = sum(len(c) for c, *_ in load)
+    traffic = traffic_source = None
+    try:  # rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled, for this launch size
+        with open(os.path.join(ROOT, "profiles", "decode_bslz4_16m_traffic.json")) as fh:
+            m = json.load(fh)
+        if m.get("chunks_per_launch") == per_step:
+            traffic = m["traffic_bytes_per_launch"]
+            traffic_source = "profiles/decode_bslz4_16m_traffic.json"
+    except (OSError, ValueError):
+        pass
+    alg = comp + per_step * f0.nbytes  # each chunk read once, each frame written once
+    achieved = alg / (kms * 1e-3) / 1e9
+    dec.close()
+    for o in outs:
+        o.free()
+    from oracle import oracle as O
+    t0 = time.perf_counter()
+    rc, out, _ = O.oracle().decompress_image(chunks[0], "bslz4", 32, h, w)
+    cpu_s = time.perf_counter() - t0
+    assert rc == 0 and out == frames_host[0].tobytes()
+    return {
+        "workload": f"{per_step} EIGER2 16M uint32 frames as bslz4 chunks "
+                    "(bitshuffle default 8192-byte blocks), resident in HBM, "
+                    "decoded into frames in HBM",
+        "value": resident, "unit": "frames/s",
+        "ms_per_step": 1e3 * per_step / resident,
+        "compression_ratio": per_step * f0.nbytes / comp,
+        "kernel_ms": kms,
+        "host_upload_included_frames_per_s": host,
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": traffic, "traffic_source": traffic_source,
+                     "kernel": "decode_bslz4 (one wave per 8192-byte block)",
+                     "bytes_per_frame": alg / per_step},
+        "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "frames/s", "cores": 1,
+                         "kind": "port",
+                         "sample": "one 16M bslz4 chunk through oracle/decompress.c"},
+    }
+
+
 def copy_calibration(sf, nbytes=1 << 30, copies=4):
     """Plain device-to-device copies: each reads and writes `nbytes`, so the
     HBM read and write counters of these dispatches give the factor from
@@ -209,6 +293,7 @@ def main():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-uneven", action="store_true")
+    ap.add_argument("--no-decompress", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only device-to-device copies of 1 GiB, the known "
                          "traffic against which HBM counters are scaled")
@@ -334,8 +419,10 @@ def main():
         # One frame per launch sequence, waiting for each: gpu::find's shape
         # with the frame already in HBM.
         ws = sf.Workspace()
-        for d in dev[:2]:
+        for d in dev[:2]:  # captures the threshold and labelling graphs
             ws.threshold(d.ptr, 32, h, w, "fused")
+            if label:
+                sf.workspace_label_enqueue(ws)
             ws.sync()
         t0 = time.perf_counter()
         for d in dev:
@@ -354,6 +441,9 @@ def main():
     if not args.no_serve:
         line["serving"] = serving(sf, args.serve_seconds, seed=2025)
 
+    if not args.no_decompress:
+        line["decompress"] = decompress(sf, frames_host)
+        log(f"decompress: {line['decompress']}")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(frames_host, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)

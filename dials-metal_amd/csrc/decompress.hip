@@ -27,6 +27,7 @@
 #include <string>
 #include <vector>
 
+#include "sf_device.hh"
 #include "sf_internal.hh"
 
 namespace {
@@ -98,7 +99,7 @@ struct Window {
     word = load(0);
   }
   __device__ uint32_t load(uint32_t b) const {
-    return *reinterpret_cast<const uint32_t *>(origin + b + 4 * lane);
+    return *sfd::gmem(reinterpret_cast<const uint32_t *>(origin + b + 4 * lane));
   }
   // Brings stream bytes [p, p + need) into the window (need <= 253).
   __device__ void hold(uint32_t p, uint32_t need) {
@@ -113,6 +114,18 @@ struct Window {
     const uint32_t d = p + shift - base;
     return (readlane(word, d >> 2) >> (8 * (d & 3))) & 0xFFu;
   }
+  // Bytes p .. p + 3, byte p lowest (those past the stream are the staging's).
+  __device__ uint32_t peek4(uint32_t p) {
+    hold(p, 4);
+    const uint32_t d = p + shift - base;
+    const uint32_t lo = readlane(word, d >> 2);
+    if ((d & 3) == 0)
+      return lo;
+    // hold(p, 4) leaves d <= 252, so a straddled word is not the last
+    const uint32_t hi = readlane(word, (d >> 2) + 1);
+    return static_cast<uint32_t>(
+        ((static_cast<uint64_t>(hi) << 32) | lo) >> (8 * (d & 3)));
+  }
   // Byte p + lane, for the lanes that hold(p, n) covered.
   __device__ uint32_t lane_byte(uint32_t p) const {
     const uint32_t d = p + shift - base + lane;
@@ -120,10 +133,14 @@ struct Window {
   }
 };
 
-// Output held whole in LDS (a bslz4 block).
+// Output held whole in LDS (a bslz4 block): `cap` bytes, then kLanes bytes
+// that lanes past the end of a copy write instead, so no write needs a branch.
 struct BlockSink {
   uint8_t *buf;
-  __device__ void put(uint32_t pos, uint32_t b) { buf[pos] = static_cast<uint8_t>(b); }
+  uint32_t cap;
+  __device__ void put(uint32_t pos, uint32_t b, bool ok, uint32_t lane) {
+    buf[ok ? pos : cap + lane] = static_cast<uint8_t>(b);
+  }
   __device__ uint32_t get(uint32_t pos) const { return buf[pos]; }
 };
 
@@ -132,9 +149,11 @@ struct BlockSink {
 struct RingSink {
   uint8_t *ring;
   uint8_t *out;
-  __device__ void put(uint32_t pos, uint32_t b) {
-    ring[pos & (kRing - 1)] = static_cast<uint8_t>(b);
-    out[pos] = static_cast<uint8_t>(b);
+  __device__ void put(uint32_t pos, uint32_t b, bool ok, uint32_t) {
+    if (ok) {
+      ring[pos & (kRing - 1)] = static_cast<uint8_t>(b);
+      sfd::gmem(out)[pos] = static_cast<uint8_t>(b);
+    }
   }
   __device__ uint32_t get(uint32_t pos) const { return ring[pos & (kRing - 1)]; }
 };
@@ -145,28 +164,42 @@ struct RingSink {
 // from op - off + i, which an earlier step (or the output before the match)
 // wrote, and within a step every lane reads before any lane writes.  In a ring
 // the slot a step writes can only be one that this step has already read.
+constexpr uint32_t kBatch = 4; // steps of 64 bytes whose reads go out together
+
 template <class Sink>
-__device__ __forceinline__ void copy_match(Sink &s, uint32_t op, uint32_t off, uint32_t len,
-                           uint32_t lane) {
+__device__ __forceinline__ void copy_match(Sink &s, uint32_t op, uint32_t off,
+                                           uint32_t len, uint32_t lane) {
+  const uint32_t steps = (len + kLanes - 1) / kLanes;
   if (off < kLanes) {
     const uint32_t lm = lane % off;
     const uint32_t pattern = s.get(op - off + lm);
     const uint32_t step = kLanes % off;
-    uint32_t r = 0; // done % off
-    for (uint32_t done = 0; done < len; done += kLanes) {
+    uint32_t r = 0; // (64 k) % off
+    for (uint32_t k = 0; k < steps; k++) {
       uint32_t m = lm + r;
       m = m >= off ? m - off : m;
-      const uint32_t b = bpermute(pattern, m);
-      if (done + lane < len)
-        s.put(op + done + lane, b);
+      const uint32_t i = k * kLanes + lane;
+      s.put(op + i, bpermute(pattern, m), i < len, lane);
       r += step;
       r = r >= off ? r - off : r;
     }
   } else {
-    for (uint32_t done = 0; done < len; done += kLanes) {
-      const uint32_t i = done + lane;
-      if (i < len)
-        s.put(op + i, s.get(op - off + i));
+    // Steps whose sources all precede this batch's first output byte: the
+    // first off / 64 of them.  (A lane past the end reads below op + len.)
+    const uint32_t m = min(kBatch, off / kLanes);
+    for (uint32_t k = 0; k < steps; k += m) {
+      const uint32_t mm = min(m, steps - k);
+      uint32_t b[kBatch];
+#pragma unroll
+      for (uint32_t j = 0; j < kBatch; j++)
+        if (j < mm)
+          b[j] = s.get(op - off + (k + j) * kLanes + lane);
+#pragma unroll
+      for (uint32_t j = 0; j < kBatch; j++)
+        if (j < mm) {
+          const uint32_t i = (k + j) * kLanes + lane;
+          s.put(op + i, b[j], i < len, lane);
+        }
     }
   }
 }
@@ -201,12 +234,14 @@ __device__ __forceinline__ int64_t lz4_block(Window &in, uint32_t n, Sink &s, ui
     }
     if (lit > n - ip || lit > cap - op)
       return -1;
-    for (uint32_t done = 0; done < lit; done += kLanes) {
-      const uint32_t cnt = min(kLanes, lit - done);
+    // Up to three steps a window: 192 bytes, all of them inside one hold.
+    for (uint32_t done = 0; done < lit; done += 3 * kLanes) {
+      const uint32_t cnt = min(3 * kLanes, lit - done);
       in.hold(ip + done, cnt);
-      const uint32_t b = in.lane_byte(ip + done);
-      if (lane < cnt)
-        s.put(op + done + lane, b);
+      for (uint32_t j = 0; j * kLanes < cnt; j++) {
+        const uint32_t i = j * kLanes + lane;
+        s.put(op + done + i, in.lane_byte(ip + done + j * kLanes), i < cnt, lane);
+      }
     }
     ip += lit;
     op += lit;
@@ -214,15 +249,19 @@ __device__ __forceinline__ int64_t lz4_block(Window &in, uint32_t n, Sink &s, ui
       return op;
     if (cap < 12 || op > cap - 12 || n - ip < 8)
       return -1;
-    const uint32_t offset = in.byte(ip) | (in.byte(ip + 1) << 8);
+    // The offset and the first two length bytes in one read (n - ip >= 8).
+    const uint32_t o4 = in.peek4(ip);
+    const uint32_t offset = o4 & 0xFFFFu;
     ip += 2;
     uint32_t len = token & 15;
     if (len == 15) {
-      uint32_t b;
+      uint32_t b, k = 2;
       do {
         if (ip >= n)
           return -1;
-        b = in.byte(ip++);
+        b = k < 4 ? (o4 >> (8 * k)) & 0xFFu : in.byte(ip);
+        ip++;
+        k++;
         len += b;
         if (len > cap)
           return -1;
@@ -296,7 +335,7 @@ __device__ __forceinline__ void unshuffle(const uint8_t *rows, uint32_t count, u
       }
     }
     transpose32(a);
-    uint64_t *o = reinterpret_cast<uint64_t *>(out + 32 * g * E);
+    SF_GLOBAL uint64_t *o = sfd::gmem(reinterpret_cast<uint64_t *>(out + 32 * g * E));
     if constexpr (E == 4) {
 #pragma unroll
       for (int t = 0; t < 16; t++)
@@ -333,9 +372,12 @@ template <uint32_t Cap, int Waves, int E>
 __global__ __launch_bounds__(Waves * kLanes) void decode_bslz4(
     const ChunkDev *__restrict__ chunks, const BlockItem *__restrict__ items,
     uint32_t n, uint32_t *status) {
-  __shared__ __align__(16) uint8_t lds[Waves][Cap];
+  __shared__ __align__(16) uint8_t lds[Waves][Cap + kLanes];
   const uint32_t lane = threadIdx.x % kLanes;
-  const uint32_t wave = threadIdx.x / kLanes;
+  // The wave's number, proven uniform, so that its item, its chunk and every
+  // position the parse derives from them live in scalar registers.
+  const uint32_t wave = static_cast<uint32_t>(
+      __builtin_amdgcn_readfirstlane(static_cast<int>(threadIdx.x / kLanes)));
   const uint32_t k = blockIdx.x * Waves + wave;
   if (k >= n)
     return;
@@ -346,14 +388,14 @@ __global__ __launch_bounds__(Waves * kLanes) void decode_bslz4(
   if (it.index == kLeftover) { // the count % 8 elements after the blocks
     const size_t at = static_cast<size_t>(c.count - c.count % 8) * c.elem;
     if (lane < it.bytes)
-      frame[at + lane] = payload[it.at + lane];
+      sfd::gmem(frame)[at + lane] = sfd::gmem(payload)[it.at + lane];
     return;
   }
   const uint32_t first = it.index * c.block;
   const uint32_t count = min(c.block, c.count - first) / 8 * 8;
   const uint32_t cap = count * E; // at most Cap (the host sorts)
   Window in(payload + it.at, lane);
-  BlockSink s{lds[wave]};
+  BlockSink s{lds[wave], cap};
   const int64_t got = lz4_block(in, it.bytes, s, cap, lane);
   if (got != static_cast<int64_t>(cap)) {
     fail(status, it.chunk, got < 0 ? kMalformed : kWrongSize, lane);

@@ -807,9 +807,9 @@ class Decoder:
         arr = (Chunk * len(chunks))()
         keep = []
         for i, (data, algo, bd, h, w, frame) in enumerate(chunks):
-            buf = ctypes.create_string_buffer(bytes(data), len(data))
-            keep.append(buf)
-            arr[i].data = ctypes.cast(buf, ctypes.c_void_p)
+            data = data if isinstance(data, bytes) else bytes(data)
+            keep.append(data)  # read in place: the library copies it in load()
+            arr[i].data = ctypes.cast(ctypes.c_char_p(data), ctypes.c_void_p)
             arr[i].size = len(data)
             arr[i].algorithm = (algo or "").encode()
             arr[i].bit_depth, arr[i].height, arr[i].width = bd, h, w
