@@ -132,6 +132,18 @@ class SynthDesc(ctypes.Structure):
                 ("persist", ctypes.c_uint32), ("threads", ctypes.c_uint32)]
 
 
+class NarrowJob(ctypes.Structure):
+    _fields_ = [("frame", ctypes.c_void_p), ("out16", ctypes.c_void_p),
+                ("pixels", ctypes.c_uint64), ("bits", ctypes.c_uint32),
+                ("reserved", ctypes.c_uint32)]
+
+
+class FrameScan(ctypes.Structure):
+    _fields_ = [("largest", ctypes.c_uint64), ("bad", ctypes.c_uint64),
+                ("joins", ctypes.c_uint64), ("over", ctypes.c_uint64),
+                ("fits", ctypes.c_int32), ("reserved", ctypes.c_int32)]
+
+
 class Chunk(ctypes.Structure):
     """sf_chunk: a chunk as HDF5 stored it, and its frame in device memory."""
     _fields_ = [("data", ctypes.c_void_p), ("size", ctypes.c_size_t),
@@ -246,6 +258,8 @@ def lib():
         "sf_synth_eiger": ([ctypes.POINTER(SynthDesc), ctypes.c_int, ctypes.c_int], None),
         "sf_synth_frame": ([ctypes.POINTER(SynthDesc), ctypes.c_int64, vp], ctypes.c_int),
         "sf_frame_bytes": ([sz, sz, ctypes.c_uint], sz),
+        "sf_narrow_frames": ([ctypes.POINTER(NarrowJob), sz, ctypes.POINTER(FrameScan),
+                              ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "sf_decompress_image": ([vp, sz, ctypes.c_char_p, ctypes.c_uint, u32, u32,
                                  vp, sz], ctypes.c_int),
         "sf_decoder_create": ([vp], vp),
@@ -839,3 +853,24 @@ class Decoder:
         ms = ctypes.c_double(0)
         _check(lib().sf_decoder_kernel_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+
+# ---- --gpu-force and mxi_max on frames in HBM ----
+
+def narrow_frames(jobs, timed=False):
+    """narrow_to_16 (find_spots.cc:162-182) and scan<T> (apps/mxi_max.cc:45-68)
+    over device frames in one launch.  jobs: (frame, out16 or None, pixels,
+    bits) with frame/out16 DeviceArrays or device addresses.  Returns a list of
+    dicts (largest, bad, joins, over, fits), and the device time in ms when
+    timed."""
+    arr = (NarrowJob * len(jobs))()
+    for i, (frame, out, pixels, bits) in enumerate(jobs):
+        arr[i].frame = frame.ptr if hasattr(frame, "ptr") else frame
+        arr[i].out16 = (out.ptr if hasattr(out, "ptr") else out) if out is not None else None
+        arr[i].pixels, arr[i].bits = pixels, bits
+    res = (FrameScan * max(len(jobs), 1))()
+    ms = ctypes.c_double(0)
+    _check(lib().sf_narrow_frames(arr, len(jobs), res, ctypes.byref(ms)))
+    out = [{k: int(getattr(r, k)) for k in ("largest", "bad", "joins", "over")}
+           | {"fits": bool(r.fits)} for r in list(res)[: len(jobs)]]
+    return (out, ms.value) if timed else out

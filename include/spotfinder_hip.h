@@ -437,6 +437,36 @@ int sf_decoder_sync(sf_decoder *d, int *status);
 void sf_decoder_set_timing(sf_decoder *d, int on);
 int sf_decoder_kernel_ms(sf_decoder *d, double *ms);
 
+/* ------------------------------------------------------------------------
+ * --gpu-force and mxi_max on frames in HBM.  narrow_to_16
+ * (src/spots/find_spots.cc:162-182): a 32-bit frame's pixels as their low 16
+ * bits, so that the markers 0xfffffffe (bad pixel) and 0xffffffff (tile join)
+ * become 0xfffe and 0xffff; it fits when no other pixel reaches 0xFFFD.
+ * scan<T> (apps/mxi_max.cc:45-68): the largest count that is not a marker,
+ * and how many pixels are each marker or a count at or above 0xFFFD.
+ * ------------------------------------------------------------------------ */
+typedef struct sf_narrow_job {
+  const void *frame; /* device memory, `pixels` pixels of `bits`, 16-byte aligned */
+  void *out16;       /* device memory for the 16-bit frame (32-bit frames), or
+                        NULL to scan only (the only choice for 16-bit frames) */
+  uint64_t pixels;
+  uint32_t bits;     /* 16 or 32 */
+  uint32_t reserved;
+} sf_narrow_job;
+
+typedef struct sf_frame_scan {
+  uint64_t largest;  /* the largest count that is not a marker */
+  uint64_t bad, joins, over; /* pixels at max() - 1, at max(), counts >= 0xFFFD */
+  int32_t fits;      /* largest < 0xFFFD: narrow_to_16's answer */
+  int32_t reserved;
+} sf_frame_scan;
+
+/* All jobs in one launch on the calling thread's stream; returns when the
+ * results are in (device_ms, if not NULL: the launch's device time).
+ * 0, -1 (bad argument), -3. */
+int sf_narrow_frames(const sf_narrow_job *jobs, size_t n, sf_frame_scan *results,
+                     double *device_ms);
+
 #ifdef __cplusplus
 }
 #endif

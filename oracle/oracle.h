@@ -7,6 +7,8 @@
  *   oracle_dext_u16 / oracle_dext_u32  src/spots/dext.cc:8-225 (dext<T>)
  *   oracle_sort_by_index              src/spots/signal_order.cc:19-72
  *   oracle_labeller_*                 src/spots/dials_spots.cc:36-360
+ *   oracle_narrow_to_16, oracle_scan_* src/spots/find_spots.cc:162-182,
+ *                                     apps/mxi_max.cc:45-68
  *   oracle_decompress_image etc.      src/spots/decompress.cc:1-136, with the
  *                                     bitshuffle and lz4 codecs it calls
  *                                     (oracle/decompress.c says how those are
@@ -110,6 +112,15 @@ int oracle_decompress_image(const uint8_t *data, size_t n, const char *algorithm
                             unsigned bit_depth, size_t height, size_t width,
                             uint8_t *out, size_t out_size, char *err,
                             size_t err_len);
+
+/* --gpu-force's narrowing and mxi_max's scan (oracle/narrow.c). */
+typedef struct {
+  uint64_t largest, bad, joins, over;
+} oracle_frame_scan;
+int oracle_narrow_to_16(const uint32_t *in, size_t n, uint16_t *out,
+                        uint32_t *offending);
+void oracle_scan_u16(const uint16_t *px, size_t n, oracle_frame_scan *s);
+void oracle_scan_u32(const uint32_t *px, size_t n, oracle_frame_scan *s);
 
 #ifdef __cplusplus
 }

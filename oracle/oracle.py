@@ -34,6 +34,10 @@ class _Counts(ctypes.Structure):
     _fields_ = [(f, ctypes.c_uint64) for f in COUNTS_FIELDS]
 
 
+class _Scan(ctypes.Structure):
+    _fields_ = [(f, ctypes.c_uint64) for f in ("largest", "bad", "joins", "over")]
+
+
 def _p(a):
     return a.ctypes.data_as(ctypes.c_void_p)
 
@@ -90,6 +94,10 @@ class _Lib:
                 vp, sz, ctypes.c_char_p, ctypes.c_uint, sz, sz, vp, sz,
                 ctypes.c_char_p, sz]
             L.oracle_decompress_image.restype = ctypes.c_int
+            L.oracle_narrow_to_16.argtypes = [vp, sz, vp, ctypes.POINTER(ctypes.c_uint32)]
+            L.oracle_narrow_to_16.restype = ctypes.c_int
+            L.oracle_scan_u16.argtypes = [vp, sz, ctypes.POINTER(_Scan)]
+            L.oracle_scan_u32.argtypes = [vp, sz, ctypes.POINTER(_Scan)]
         else:
             L.ref_labeller_spots.argtypes = [vp, vp]
             L.ref_labeller_pixels.argtypes = [vp, vp]
@@ -163,6 +171,25 @@ class _Lib:
         if rc != 0:
             return rc, None, err.value.decode()
         return 0, out[:expected].tobytes(), ""
+
+    # ---- --gpu-force's narrowing and mxi_max's scan ----
+
+    def narrow_to_16(self, frame):
+        """(fits, offending, uint16 frame) for a uint32 frame."""
+        f = np.ascontiguousarray(frame, np.uint32)
+        out = np.zeros(f.shape, np.uint16)
+        worst = ctypes.c_uint32(0)
+        ok = self.lib.oracle_narrow_to_16(_p(f), f.size, _p(out), ctypes.byref(worst))
+        return bool(ok), worst.value, out
+
+    def scan(self, frame):
+        """dict(largest, bad, joins, over) for a uint16 or uint32 frame."""
+        f = np.ascontiguousarray(frame)
+        s = _Scan()
+        fn = {np.dtype(np.uint16): self.lib.oracle_scan_u16,
+              np.dtype(np.uint32): self.lib.oracle_scan_u32}[f.dtype]
+        fn(_p(f), f.size, ctypes.byref(s))
+        return {k: int(getattr(s, k)) for k in ("largest", "bad", "joins", "over")}
 
     def labeller(self, height, width, min_spot_size=3, max_spot_size=1000,
                  max_separation=2.0, two_d=False):
