@@ -19,7 +19,8 @@ its stream, and the reference's own CPU threshold (compiled from its
 sources, oracle/_ref) on this host's cores over a bounded sample.  The
 `decompress` object measures the step before the path: 16M frames stored as
 bslz4 chunks (the EIGER's bitshuffle-LZ4 HDF5 filter), resident in HBM and
-decoded into frames on the device.
+decoded into frames on the device.  The `narrow` object measures
+--gpu-force's narrowing of those frames to 16 bits with mxi_max's scan.
 """
 import argparse
 import json
@@ -255,6 +256,60 @@ def decompress(sf, frames_host, distinct=4, per_step=32, steps=10):
     }
 
 
+def narrow(sf, frames_host, per_step=32, steps=10):
+    """--gpu-force's narrowing of 32-bit frames to 16 bits with mxi_max's
+    scan, on `per_step` 16M frames resident in HBM in one launch: HBM bytes 4
+    read and 2 written a pixel.  The oracle's narrow_to_16 on one frame on one
+    host thread beside it."""
+    f0 = frames_host[0]
+    dev = [sf.DeviceArray.from_numpy(frames_host[i % len(frames_host)])
+           for i in range(per_step)]
+    outs = [sf.DeviceArray(2 * f0.size) for _ in range(per_step)]
+    jobs = [(d, o, f0.size, 32) for d, o in zip(dev, outs)]
+    res = sf.narrow_frames(jobs)
+    from oracle import oracle as O
+    fits, worst, f16 = O.oracle().narrow_to_16(frames_host[0])
+    if res[0]["largest"] != worst or outs[0].to_numpy(np.uint16, f0.shape).tobytes() != f16.tobytes():
+        raise RuntimeError("the device narrowing differs from the oracle's")
+    ms = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        ms.append(sf.narrow_frames(jobs, timed=True)[1])
+    wall = (time.perf_counter() - t0) / steps
+    kms = float(np.median(ms))
+    alg = 6.0 * f0.size * per_step
+    achieved = alg / (kms * 1e-3) / 1e9
+    traffic = traffic_source = None
+    try:  # rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled, for this launch size
+        with open(os.path.join(ROOT, "profiles", "narrow_scan_16m_traffic.json")) as fh:
+            m = json.load(fh)
+        if m.get("frames_per_launch") == per_step:
+            traffic = m["traffic_bytes_per_launch"]
+            traffic_source = "profiles/narrow_scan_16m_traffic.json"
+    except (OSError, ValueError):
+        pass
+    t0 = time.perf_counter()
+    O.oracle().narrow_to_16(frames_host[0])
+    cpu_s = time.perf_counter() - t0
+    for x in dev + outs:
+        x.free()
+    return {
+        "workload": f"{per_step} EIGER2 16M uint32 frames resident in HBM, narrowed "
+                    "to uint16 with the largest valid count, marker and over-limit "
+                    "counts, one launch",
+        "value": per_step / (kms * 1e-3), "unit": "frames/s",
+        "wall_frames_per_s": per_step / wall, "kernel_ms": kms,
+        "fits_16_bits": bool(res[0]["fits"]), "largest_count": res[0]["largest"],
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": traffic,
+                     "traffic_source": traffic_source,
+                     "kernel": "narrow_scan", "bytes_per_frame": alg / per_step},
+        "cpu_baseline": {"value": 1.0 / cpu_s, "unit": "frames/s", "cores": 1,
+                         "kind": "port",
+                         "sample": "one 16M frame through oracle/narrow.c"},
+    }
+
+
 def copy_calibration(sf, nbytes=1 << 30, copies=4):
     """Plain device-to-device copies: each reads and writes `nbytes`, so the
     HBM read and write counters of these dispatches give the factor from
@@ -294,6 +349,7 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=2.0)
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-decompress", action="store_true")
+    ap.add_argument("--no-narrow", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only device-to-device copies of 1 GiB, the known "
                          "traffic against which HBM counters are scaled")
@@ -444,6 +500,9 @@ def main():
     if not args.no_decompress:
         line["decompress"] = decompress(sf, frames_host)
         log(f"decompress: {line['decompress']}")
+    if not args.no_narrow:
+        line["narrow"] = narrow(sf, frames_host)
+        log(f"narrow: {line['narrow']}")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(frames_host, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)
