@@ -950,4 +950,13 @@ sf_counts sf_labeller_counts(const sf_labeller *l) {
 }
 size_t sf_labeller_live(const sf_labeller *l) { return l ? l->lab.live() : 0; }
 
+int sf_labeller_write_refl(const sf_labeller *l, const char *path,
+                           const sf_refl_options *options) {
+  if (l == nullptr)
+    return bad("labeller");
+  return sf_refl_write(path, l->lab.spots().data(), l->lab.spots().size(),
+                       l->lab.pixels().data(), l->lab.pixels().size(),
+                       l->lab.width(), options);
+}
+
 } // extern "C"

@@ -144,6 +144,11 @@ class FrameScan(ctypes.Structure):
                 ("fits", ctypes.c_int32), ("reserved", ctypes.c_int32)]
 
 
+class ReflOptions(ctypes.Structure):
+    _fields_ = [("panel", ctypes.c_uint64), ("id", ctypes.c_int32),
+                ("shoeboxes", ctypes.c_int32), ("identifier", ctypes.c_char_p)]
+
+
 class Chunk(ctypes.Structure):
     """sf_chunk: a chunk as HDF5 stored it, and its frame in device memory."""
     _fields_ = [("data", ctypes.c_void_p), ("size", ctypes.c_size_t),
@@ -258,6 +263,13 @@ def lib():
         "sf_synth_eiger": ([ctypes.POINTER(SynthDesc), ctypes.c_int, ctypes.c_int], None),
         "sf_synth_frame": ([ctypes.POINTER(SynthDesc), ctypes.c_int64, vp], ctypes.c_int),
         "sf_frame_bytes": ([sz, sz, ctypes.c_uint], sz),
+        "sf_refl_options_default": ([ctypes.POINTER(ReflOptions)], None),
+        "sf_refl_write": ([ctypes.c_char_p, vp, sz, vp, sz, sz,
+                           ctypes.POINTER(ReflOptions)], ctypes.c_int),
+        "sf_refl_encode": ([vp, sz, vp, sz, sz, ctypes.POINTER(ReflOptions), vp, sz,
+                            ctypes.POINTER(sz)], ctypes.c_int),
+        "sf_labeller_write_refl": ([vp, ctypes.c_char_p, ctypes.POINTER(ReflOptions)],
+                                   ctypes.c_int),
         "sf_narrow_frames": ([ctypes.POINTER(NarrowJob), sz, ctypes.POINTER(FrameScan),
                               ctypes.POINTER(ctypes.c_double)], ctypes.c_int),
         "sf_decompress_image": ([vp, sz, ctypes.c_char_p, ctypes.c_uint, u32, u32,
@@ -657,6 +669,13 @@ class Labeller:
     def live(self):
         return int(lib().sf_labeller_live(self.h))
 
+    def write_refl(self, path, panel=0, id=0, identifier="", shoeboxes=True):
+        """The finished grouping's spots as a DIALS reflection table
+        (refl::write, src/spots/refl.hh:70-73)."""
+        o = _refl_options(panel, id, identifier, shoeboxes)
+        if lib().sf_labeller_write_refl(self.h, str(path).encode(), ctypes.byref(o)) != 0:
+            raise RuntimeError(last_error())
+
 
 def autotune(frames, label=True):
     """sf_autotune over device frames given as (ptr, height, width, bits):
@@ -874,3 +893,40 @@ def narrow_frames(jobs, timed=False):
     out = [{k: int(getattr(r, k)) for k in ("largest", "bad", "joins", "over")}
            | {"fits": bool(r.fits)} for r in list(res)[: len(jobs)]]
     return (out, ms.value) if timed else out
+
+
+# ---- the spots as a DIALS reflection table (src/spots/refl.cc:223-351) ----
+
+def _refl_options(panel, id, identifier, shoeboxes):
+    o = ReflOptions()
+    lib().sf_refl_options_default(ctypes.byref(o))
+    o.panel, o.id, o.shoeboxes = panel, id, int(bool(shoeboxes))
+    o.identifier = identifier.encode() if identifier else None
+    return o
+
+
+def refl_encode(spots, pixels, width, panel=0, id=0, identifier="", shoeboxes=True):
+    """The .refl bytes of SPOT / PIXEL arrays (a grouping's output)."""
+    spots = np.ascontiguousarray(spots, SPOT)
+    pixels = np.ascontiguousarray(pixels, PIXEL)
+    o = _refl_options(panel, id, identifier, shoeboxes)
+    size = ctypes.c_size_t(0)
+    args = (_ptr(spots) if len(spots) else None, len(spots),
+            _ptr(pixels) if len(pixels) else None, len(pixels), width, ctypes.byref(o))
+    rc = lib().sf_refl_encode(*args, None, 0, ctypes.byref(size))
+    if rc not in (0, -2):
+        raise RuntimeError(last_error())
+    out = np.zeros(max(size.value, 1), np.uint8)
+    if lib().sf_refl_encode(*args, _ptr(out), size.value, ctypes.byref(size)) != 0:
+        raise RuntimeError(last_error())
+    return out[: size.value].tobytes()
+
+
+def refl_write(path, spots, pixels, width, panel=0, id=0, identifier="", shoeboxes=True):
+    spots = np.ascontiguousarray(spots, SPOT)
+    pixels = np.ascontiguousarray(pixels, PIXEL)
+    o = _refl_options(panel, id, identifier, shoeboxes)
+    if lib().sf_refl_write(str(path).encode(), _ptr(spots) if len(spots) else None,
+                           len(spots), _ptr(pixels) if len(pixels) else None,
+                           len(pixels), width, ctypes.byref(o)) != 0:
+        raise RuntimeError(last_error())

@@ -438,6 +438,34 @@ void sf_decoder_set_timing(sf_decoder *d, int on);
 int sf_decoder_kernel_ms(sf_decoder *d, double *ms);
 
 /* ------------------------------------------------------------------------
+ * The spots as a DIALS reflection table: refl::write (src/spots/refl.hh:20-78,
+ * refl.cc:223-351), byte for byte.  One msgpack document of the columns
+ * dials.find_spots writes (bbox, flags, id, intensity.sum.value and
+ * .variance, n_signal, panel, optionally shoebox, xyzobs.px.value and
+ * .variance).  `spots` and `pixels` are the grouping's output and `width` the
+ * frame width it was given.  -1 with sf_last_error() when the file cannot be
+ * written or the spots are inconsistent; sf_refl_encode returns -2 when the
+ * table (*size bytes) does not fit in `capacity`.
+ * ------------------------------------------------------------------------ */
+typedef struct sf_refl_options {
+  uint64_t panel;          /* one panel (refl.hh: Options::panel) */
+  int32_t id;              /* the experiment every spot belongs to */
+  int32_t shoeboxes;       /* 1: write the shoebox column */
+  const char *identifier;  /* the experiment identifier, or NULL / "" */
+} sf_refl_options;
+
+void sf_refl_options_default(sf_refl_options *o); /* 0, 0, shoeboxes, none */
+int sf_refl_write(const char *path, const sf_spot *spots, size_t nspots,
+                  const sf_pixel *pixels, size_t npixels, size_t width,
+                  const sf_refl_options *options);
+int sf_refl_encode(const sf_spot *spots, size_t nspots, const sf_pixel *pixels,
+                   size_t npixels, size_t width, const sf_refl_options *options,
+                   uint8_t *out, size_t capacity, size_t *size);
+/* A finished grouping's spots, written as above with its own frame width. */
+int sf_labeller_write_refl(const sf_labeller *l, const char *path,
+                           const sf_refl_options *options);
+
+/* ------------------------------------------------------------------------
  * --gpu-force and mxi_max on frames in HBM.  narrow_to_16
  * (src/spots/find_spots.cc:162-182): a 32-bit frame's pixels as their low 16
  * bits, so that the markers 0xfffffffe (bad pixel) and 0xffffffff (tile join)

@@ -103,6 +103,10 @@ class _Lib:
             L.ref_labeller_pixels.argtypes = [vp, vp]
             L.ref_labeller_counts.argtypes = [vp, ctypes.POINTER(_Counts)]
             L.ref_sizeof_spot.restype = sz
+            L.ref_refl_write.argtypes = [ctypes.c_char_p, vp, sz, vp, sz, sz, sz,
+                                         ctypes.c_int, ctypes.c_char_p, ctypes.c_int,
+                                         ctypes.c_char_p, sz]
+            L.ref_refl_write.restype = ctypes.c_int
 
     def dext(self, image, capacity=None):
         """(status, SignalPixel array) for a 2-D uint16 or uint32 frame."""
@@ -171,6 +175,25 @@ class _Lib:
         if rc != 0:
             return rc, None, err.value.decode()
         return 0, out[:expected].tobytes(), ""
+
+    def refl_bytes(self, spots, pixels, width, panel=0, id=0, identifier="",
+                   shoeboxes=True):
+        """The reference's refl::write of SPOT / PIXEL arrays, as bytes
+        (the compiled reference only)."""
+        import tempfile
+        spots = np.ascontiguousarray(spots, SPOT)
+        pixels = np.ascontiguousarray(pixels, PIXEL)
+        err = ctypes.create_string_buffer(256)
+        with tempfile.TemporaryDirectory() as d:
+            path = os.path.join(d, "t.refl")
+            rc = self.lib.ref_refl_write(
+                path.encode(), _p(spots) if len(spots) else None, len(spots),
+                _p(pixels) if len(pixels) else None, len(pixels), width, panel,
+                id, identifier.encode(), int(shoeboxes), err, len(err))
+            if rc != 0:
+                raise RuntimeError(err.value.decode())
+            with open(path, "rb") as fh:
+                return fh.read()
 
     # ---- --gpu-force's narrowing and mxi_max's scan ----
 

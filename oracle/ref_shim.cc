@@ -1,7 +1,7 @@
 // A C surface over the reference's own CPU spot-finding code, so the tests can
 // load it with ctypes and pin the oracle against it.  This file is ours; the
 // code it calls is compiled from the reference's sources where they lie
-// (src/spots/dext.cc, signal_order.cc, dials_spots.cc) by oracle/Makefile into
+// (src/spots/dext.cc, signal_order.cc, dials_spots.cc, refl.cc) by oracle/Makefile into
 // oracle/_ref/libspots_ref.so.  Test infrastructure only.
 #include <atomic>
 #include <chrono>
@@ -13,6 +13,7 @@
 
 #include "dext.hh"
 #include "dials_spots.hh"
+#include "refl.hh"
 #include "signal_order.hh"
 #include "signal_pixel.hh"
 
@@ -126,6 +127,28 @@ std::size_t ref_labeller_live(void *l) {
 }
 
 std::size_t ref_sizeof_spot() { return sizeof(dials_spots::Spot); }
+
+// refl::write (refl.cc:223-351) of n spots and their pixels to `path`.
+int ref_refl_write(const char *path, const dials_spots::Spot *spots,
+                   std::size_t n, const dials_spots::Pixel *pixels,
+                   std::size_t npixels, std::size_t width, std::size_t panel,
+                   int id, const char *identifier, int shoeboxes, char *err,
+                   std::size_t errlen) {
+  try {
+    const std::vector<dials_spots::Spot> s(spots, spots + n);
+    const std::vector<dials_spots::Pixel> p(pixels, pixels + npixels);
+    refl::Options o;
+    o.panel = panel;
+    o.id = id;
+    o.identifier = identifier ? identifier : "";
+    o.shoeboxes = shoeboxes != 0;
+    refl::write(path, s, p, width, o);
+    return 0;
+  } catch (const std::exception &e) {
+    copy_message(e, err, errlen);
+    return -1;
+  }
+}
 
 // The reference's threshold on N threads, one frame each at a time, with a
 // per-thread scratch as mxi_find keeps one (find_spots.cc:404-405).
