@@ -20,7 +20,8 @@ sources, oracle/_ref) on this host's cores over a bounded sample.  The
 `decompress` object measures the step before the path: 16M frames stored as
 bslz4 chunks (the EIGER's bitshuffle-LZ4 HDF5 filter), resident in HBM and
 decoded into frames on the device.  The `narrow` object measures
---gpu-force's narrowing of those frames to 16 bits with mxi_max's scan.
+--gpu-force's narrowing of those frames to 16 bits with mxi_max's scan, and
+the `refl` object the reflection-table writer beside the reference's.
 """
 import argparse
 import json
@@ -310,6 +311,38 @@ def narrow(sf, frames_host, per_step=32, steps=10):
     }
 
 
+def refl(sf, repeat=64):
+    """The spots as a DIALS reflection table with shoeboxes: the reference
+    Labeller's 2D spots of the golden sweep repeated `repeat` times, encoded by
+    this library and by the reference's own refl::write (oracle/_ref), on one
+    host thread each."""
+    z = np.load(os.path.join(ROOT, "tests", "golden", "labeller_sweep.npz"))
+    spots = np.resize(z["spots_2d"], len(z["spots_2d"]) * repeat)
+    pixels, width = z["pixels_2d"], int(z["desc_width"])
+    secs = float("inf")
+    for _ in range(3):  # the best of three, for both writers
+        t0 = time.perf_counter()
+        ours = sf.refl_encode(spots, pixels, width)
+        secs = min(secs, time.perf_counter() - t0)
+    out = {"workload": f"{len(spots)} spots with shoeboxes (the golden sweep's "
+                       f"2D spots, repeated), encoded in memory",
+           "value": len(spots) / secs, "unit": "spots/s", "bytes": len(ours),
+           "MB_per_s": len(ours) / secs / 1e6}
+    from oracle import oracle as O
+    ref = O.reference()
+    if ref is not None:
+        rsecs = float("inf")
+        for _ in range(3):
+            t0 = time.perf_counter()
+            theirs = ref.refl_bytes(spots, pixels, width)
+            rsecs = min(rsecs, time.perf_counter() - t0)
+        out["cpu_baseline"] = {"value": len(spots) / rsecs, "unit": "spots/s",
+                               "cores": 1, "kind": "reference",
+                               "sample": "the same spots through refl::write to a file",
+                               "identical": theirs == ours}
+    return out
+
+
 def copy_calibration(sf, nbytes=1 << 30, copies=4):
     """Plain device-to-device copies: each reads and writes `nbytes`, so the
     HBM read and write counters of these dispatches give the factor from
@@ -350,6 +383,7 @@ def main():
     ap.add_argument("--no-uneven", action="store_true")
     ap.add_argument("--no-decompress", action="store_true")
     ap.add_argument("--no-narrow", action="store_true")
+    ap.add_argument("--no-refl", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only device-to-device copies of 1 GiB, the known "
                          "traffic against which HBM counters are scaled")
@@ -503,6 +537,9 @@ def main():
     if not args.no_narrow:
         line["narrow"] = narrow(sf, frames_host)
         log(f"narrow: {line['narrow']}")
+    if not args.no_refl:
+        line["refl"] = refl(sf)
+        log(f"refl: {line['refl']}")
     if not args.no_cpu:
         line["cpu_baseline"] = cpu_baseline(frames_host, min(16, os.cpu_count() or 1))
     print(json.dumps(line), flush=True)

@@ -44,20 +44,30 @@ constexpr uint8_t kMaskForeground = 1u << 2;
 constexpr uint64_t kFlagStrong = 1u << 5;    // dials::af::Flags::Strong
 constexpr uint8_t kShoeboxVersion = 2;       // a uint8 mask
 
-// Where the encoded bytes go: memory, or a file through a 1 MB buffer.
+// Where the encoded bytes go: the caller's memory, or a file through a 1 MB
+// buffer.
 class Sink {
 public:
   virtual ~Sink() = default;
   virtual void put(const void *p, size_t n) = 0;
 };
 
-class MemorySink final : public Sink {
+// The caller's buffer: bytes past its capacity are counted, not written.
+class SpanSink final : public Sink {
 public:
-  std::vector<uint8_t> bytes;
+  SpanSink(uint8_t *out, size_t capacity) : out_(out), cap_(capacity) {}
   void put(const void *p, size_t n) override {
-    const uint8_t *b = static_cast<const uint8_t *>(p);
-    bytes.insert(bytes.end(), b, b + n);
+    if (size_ + n <= cap_)
+      std::memcpy(out_ + size_, p, n);
+    else if (size_ < cap_)
+      std::memcpy(out_ + size_, p, cap_ - size_);
+    size_ += n;
   }
+  size_t size() const { return size_; }
+
+private:
+  uint8_t *out_;
+  size_t cap_, size_ = 0;
 };
 
 class FileSink final : public Sink {
@@ -170,6 +180,21 @@ private:
   Sink &s_;
 };
 
+// A fixed-width column's rows, `bytes_per_row` each from fill(i, dst), staged
+// through a local buffer so that the sink sees a few large writes.
+template <typename Fill>
+void column_rows(Encoder &e, size_t n, size_t bytes_per_row, Fill fill) {
+  constexpr size_t kRows = 4096;
+  uint8_t stage[kRows * 24];
+  for (size_t i = 0; i < n;) {
+    const size_t take = std::min(kRows, n - i);
+    for (size_t k = 0; k < take; k++)
+      fill(i + k, stage + k * bytes_per_row);
+    e.bytes(stage, take * bytes_per_row);
+    i += take;
+  }
+}
+
 uint64_t box_elements(const sf_spot &s) {
   return static_cast<uint64_t>(s.bbox[1] - s.bbox[0]) *
          static_cast<uint64_t>(s.bbox[3] - s.bbox[2]) *
@@ -206,27 +231,26 @@ void write_table(Sink &sink, const sf_spot *spots, size_t n, const sf_pixel *pix
   e.str("data");
   e.map(o.shoeboxes ? 10 : 9);
 
+  auto put = [](uint8_t *dst, const void *src, size_t k) { std::memcpy(dst, src, k); };
   e.column("bbox", "int6", n, 24ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.bytes(spots[i].bbox, 24);
+  column_rows(e, n, 24, [&](size_t i, uint8_t *d) { put(d, spots[i].bbox, 24); });
   e.column("flags", "std::size_t", n, 8ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.raw<uint64_t>(kFlagStrong);
+  column_rows(e, n, 8, [&](size_t, uint8_t *d) { put(d, &kFlagStrong, 8); });
   e.column("id", "int", n, 4ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.raw<int32_t>(o.id);
+  column_rows(e, n, 4, [&](size_t, uint8_t *d) { put(d, &o.id, 4); });
   e.column("intensity.sum.value", "double", n, 8ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.raw<double>(spots[i].intensity);
+  column_rows(e, n, 8, [&](size_t i, uint8_t *d) { put(d, &spots[i].intensity, 8); });
   e.column("intensity.sum.variance", "double", n, 8ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.raw<double>(spots[i].intensity_variance);
+  column_rows(e, n, 8, [&](size_t i, uint8_t *d) {
+    put(d, &spots[i].intensity_variance, 8);
+  });
   e.column("n_signal", "int", n, 4ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.raw<int32_t>(static_cast<int32_t>(spots[i].n_signal));
+  column_rows(e, n, 4, [&](size_t i, uint8_t *d) {
+    const int32_t v = static_cast<int32_t>(spots[i].n_signal);
+    put(d, &v, 4);
+  });
   e.column("panel", "std::size_t", n, 8ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.raw<uint64_t>(o.panel);
+  column_rows(e, n, 8, [&](size_t, uint8_t *d) { put(d, &o.panel, 8); });
 
   if (o.shoeboxes) {
     uint64_t total = 0;
@@ -267,11 +291,9 @@ void write_table(Sink &sink, const sf_spot *spots, size_t n, const sf_pixel *pix
   }
 
   e.column("xyzobs.px.value", "vec3<double>", n, 24ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.bytes(spots[i].position, 24);
+  column_rows(e, n, 24, [&](size_t i, uint8_t *d) { put(d, spots[i].position, 24); });
   e.column("xyzobs.px.variance", "vec3<double>", n, 24ull * n);
-  for (size_t i = 0; i < n; i++)
-    e.bytes(spots[i].variance, 24);
+  column_rows(e, n, 24, [&](size_t i, uint8_t *d) { put(d, spots[i].variance, 24); });
 }
 
 int failed(const std::exception &ex) {
@@ -321,15 +343,11 @@ int sf_refl_encode(const sf_spot *spots, size_t nspots, const sf_pixel *pixels,
     sf_refl_options_default(&o);
     if (options)
       o = *options;
-    MemorySink m;
+    SpanSink m(out, capacity);
     write_table(m, spots, nspots, pixels, npixels, width, o);
     if (size)
-      *size = m.bytes.size();
-    if (m.bytes.size() > capacity)
-      return -2;
-    if (!m.bytes.empty())
-      std::memcpy(out, m.bytes.data(), m.bytes.size());
-    return 0;
+      *size = m.size();
+    return m.size() > capacity ? -2 : 0;
   } catch (const std::exception &ex) {
     return failed(ex);
   }

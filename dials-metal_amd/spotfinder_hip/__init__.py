@@ -913,11 +913,19 @@ def refl_encode(spots, pixels, width, panel=0, id=0, identifier="", shoeboxes=Tr
     size = ctypes.c_size_t(0)
     args = (_ptr(spots) if len(spots) else None, len(spots),
             _ptr(pixels) if len(pixels) else None, len(pixels), width, ctypes.byref(o))
-    rc = lib().sf_refl_encode(*args, None, 0, ctypes.byref(size))
-    if rc not in (0, -2):
-        raise RuntimeError(last_error())
-    out = np.zeros(max(size.value, 1), np.uint8)
-    if lib().sf_refl_encode(*args, _ptr(out), size.value, ctypes.byref(size)) != 0:
+    # At most: the headers, 112 bytes a row of plain columns, and each shoebox
+    # (29 bytes and 9 a voxel); one call writes it all.
+    cap = 4096 + len(identifier or "") + 112 * len(spots)
+    if shoeboxes and len(spots):
+        b = spots["bbox"].astype(np.int64)
+        cap += int((29 + 9 * (b[:, 1] - b[:, 0]) * (b[:, 3] - b[:, 2])
+                    * (b[:, 5] - b[:, 4])).sum())
+    out = np.empty(cap, np.uint8)
+    rc = lib().sf_refl_encode(*args, _ptr(out), cap, ctypes.byref(size))
+    if rc == -2:  # not reached for a consistent table; sized exactly, once more
+        out = np.empty(size.value, np.uint8)
+        rc = lib().sf_refl_encode(*args, _ptr(out), size.value, ctypes.byref(size))
+    if rc != 0:
         raise RuntimeError(last_error())
     return out[: size.value].tobytes()
 
