@@ -855,14 +855,17 @@ uint32_t strips_of(uint32_t width, int lane_columns) {
 }
 
 uint32_t segment_rows(uint64_t strip_rows, const Tuning &t) {
-  if (t.seg_rows > 0)
-    return static_cast<uint32_t>(t.seg_rows);
   // Segments tall enough that the 20 halo rows cost little, short enough
-  // that the batch still spreads over the whole device: about target_waves
+  // that the launch still spreads over the whole device: about target_waves
   // items, each 32 to 256 rows.
   const uint64_t target = static_cast<uint64_t>(t.target_waves);
   uint64_t seg = (strip_rows + target - 1) / target;
   seg = seg < 32 ? 32 : (seg > 256 ? 256 : seg);
+  // A set length is a ceiling: a launch too small to fill the device with
+  // segments that long (a lone frame, under a shape tuned for batches) takes
+  // the shorter ones.
+  if (t.seg_rows > 0 && static_cast<uint64_t>(t.seg_rows) < seg)
+    seg = static_cast<uint64_t>(t.seg_rows);
   return static_cast<uint32_t>(seg);
 }
 

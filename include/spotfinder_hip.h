@@ -117,8 +117,10 @@ int sf_get_kernel(void);
 
 /* The fused kernel's launch shape, which the autotuner writes.  A wave sweeps
  * a strip of 64 x `lane_columns` columns (4 or 8 a lane) down `seg_rows` rows
- * (plus 20 rows of halo); seg_rows 0 sizes segments so that one frame makes
- * about `target_waves` waves.  `waves_per_block` is 1, 2 or 4;
+ * (plus 20 rows of halo).  Segments are sized so that a launch makes about
+ * `target_waves` waves, 32 to 256 rows each; a non-zero seg_rows caps that
+ * length (so a lone frame under a shape tuned for batches still spreads over
+ * the device).  `waves_per_block` is 1, 2 or 4;
  * `frames_per_launch` is the batched mode's batch.  No counterpart in the
  * reference, whose launch shape is fixed (dext_cuda.cu:728-730).  From
  * SPOTFINDER_HIP_TUNE, e.g. "seg_rows=128,target_waves=4096,
