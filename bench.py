@@ -311,6 +311,40 @@ def narrow(sf, frames_host, per_step=32, steps=10):
     }
 
 
+def pipeline_3d(sf, dev, h, w, batches=4):
+    """The whole spot-finding chain in 3D: `batches` steps of the benchmark's
+    32 resident frames (threshold, ordering and labelling on the device), each
+    batch's components stitched across frames into spots by the host grouping
+    (sf_labeller_add_batch), as consecutive frames of one sweep."""
+    descs = [(d.ptr, h, w, 32) for d in dev]
+    b = sf.Batch()
+    b.threshold(descs, graph=True)
+    b.label(True)
+    b.sync()
+    lab = sf.Labeller(h, w)
+    t_dev = t_host = 0.0
+    for step in range(batches):
+        t0 = time.perf_counter()
+        b.threshold(descs, graph=True)
+        b.label(True)
+        b.sync()
+        t1 = time.perf_counter()
+        lab.add_batch(step * len(dev), b)
+        t_dev += t1 - t0
+        t_host += time.perf_counter() - t1
+    t0 = time.perf_counter()
+    lab.finish()
+    t_host += time.perf_counter() - t0
+    n = batches * len(dev)
+    b.close()
+    return {"workload": f"{n} consecutive 16M frames (the benchmark's 32, cycled) "
+                        "through threshold, labelling and 3D grouping into spots",
+            "value": n / (t_dev + t_host), "unit": "frames/s",
+            "device_ms_per_frame": 1e3 * t_dev / n,
+            "host_grouping_ms_per_frame": 1e3 * t_host / n,
+            "spots": len(lab.spots()), "counts": lab.counts()}
+
+
 def refl(sf, repeat=64):
     """The spots as a DIALS reflection table with shoeboxes: the reference
     Labeller's 2D spots of the golden sweep repeated `repeat` times, encoded by
@@ -384,6 +418,7 @@ def main():
     ap.add_argument("--no-decompress", action="store_true")
     ap.add_argument("--no-narrow", action="store_true")
     ap.add_argument("--no-refl", action="store_true")
+    ap.add_argument("--no-pipeline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only device-to-device copies of 1 GiB, the known "
                          "traffic against which HBM counters are scaled")
@@ -537,6 +572,9 @@ def main():
     if not args.no_narrow:
         line["narrow"] = narrow(sf, frames_host)
         log(f"narrow: {line['narrow']}")
+    if not args.no_pipeline:
+        line["pipeline_3d"] = pipeline_3d(sf, dev, h, w)
+        log(f"pipeline_3d: {line['pipeline_3d']}")
     if not args.no_refl:
         line["refl"] = refl(sf)
         log(f"refl: {line['refl']}")

@@ -26,6 +26,8 @@
 // own final operations.  The variance's sum of value * delta^2 is formed from
 // the moments (N = S Q - W^2, exact in 128 bits) rather than summed pixel by
 // pixel, so it can differ from the reference's rounding in the last bits.
+#include <sys/mman.h>
+
 #include <algorithm>
 #include <cmath>
 #include <cstring>
@@ -60,6 +62,25 @@ double to_double(u128 v) {
 
 } // namespace
 
+template <typename T> void BigArray<T>::grow(size_t need) {
+  constexpr size_t kHuge = 2u << 20;
+  size_t cap = std::max<size_t>({need, 2 * cap_, kHuge / sizeof(T)});
+  const size_t bytes = (cap * sizeof(T) + kHuge - 1) / kHuge * kHuge;
+  cap = bytes / sizeof(T);
+  T *next = static_cast<T *>(std::aligned_alloc(kHuge, bytes));
+  if (!next)
+    throw std::bad_alloc();
+  (void)madvise(next, bytes, MADV_HUGEPAGE); // a hint: small pages otherwise
+  if (size_)
+    std::memcpy(next, data_, size_ * sizeof(T));
+  std::free(data_);
+  data_ = next;
+  cap_ = cap;
+}
+
+template class BigArray<sf_pixel>;
+template class BigArray<sf_spot>;
+
 Labeller::Labeller(size_t height, size_t width, const Options &options)
     : height_(height), width_(width), options_(options) {
   if (height == 0 || width == 0)
@@ -78,16 +99,27 @@ void Labeller::check_frame(int64_t frame, const sf_signal_pixel *signal,
                              " arrived after frame " +
                              std::to_string(last_frame_) +
                              "; frames must be added in ascending order");
+  if (n == 0)
+    return;
+  // One pass without branches; the error, if any, is found afterwards. An
+  // ascending list is in range when its last index is.
   const size_t pixels = height_ * width_;
-  for (size_t i = 0; i < n; i++) {
-    if (signal[i].index >= pixels)
-      throw std::runtime_error("dials_spots: signal pixel " +
-                               std::to_string(signal[i].index) +
-                               " is off the end of the frame");
-    if (i > 0 && signal[i].index <= signal[i - 1].index)
-      throw std::runtime_error(
-          "dials_spots: signal pixels are not ascending by index");
+  bool descends = false;
+  for (size_t i = 1; i < n; i++)
+    descends |= signal[i].index <= signal[i - 1].index;
+  if (descends) {
+    for (size_t i = 0; i < n; i++)
+      if (signal[i].index >= pixels)
+        throw std::runtime_error("dials_spots: signal pixel " +
+                                 std::to_string(signal[i].index) +
+                                 " is off the end of the frame");
+    throw std::runtime_error(
+        "dials_spots: signal pixels are not ascending by index");
   }
+  if (signal[n - 1].index >= pixels)
+    throw std::runtime_error("dials_spots: signal pixel " +
+                             std::to_string(signal[n - 1].index) +
+                             " is off the end of the frame");
 }
 
 uint32_t Labeller::find(uint32_t i) {
@@ -118,17 +150,25 @@ void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
 
   // Pixels of this frame grouped by component, each group still ascending.
   Arena &arena = arenas_[frame];
+  if (!spare_arenas_.empty()) { // a released frame's buffers, capacity kept
+    arena = std::move(spare_arenas_.back());
+    spare_arenas_.pop_back();
+  }
   arena.index.resize(n);
   arena.value.resize(n);
-  std::vector<uint32_t> start(ncomp + 1, 0);
-  for (size_t i = 0; i < n; i++) {
-    if (comp_of[i] >= ncomp)
-      throw std::runtime_error("device labelling: component out of range");
+  std::vector<uint32_t> &start = start_;
+  start.assign(ncomp + 1, 0);
+  uint32_t top = 0;
+  for (size_t i = 0; i < n; i++)
+    top = std::max(top, comp_of[i]);
+  if (n && top >= ncomp)
+    throw std::runtime_error("device labelling: component out of range");
+  for (size_t i = 0; i < n; i++)
     start[comp_of[i] + 1]++;
-  }
   for (size_t c = 0; c < ncomp; c++)
     start[c + 1] += start[c];
-  std::vector<uint32_t> at(start.begin(), start.end() - 1);
+  std::vector<uint32_t> &at = at_;
+  at.assign(start.begin(), start.end() - 1);
   for (size_t i = 0; i < n; i++) {
     const uint32_t k = at[comp_of[i]]++;
     arena.index[k] = signal[i].index;
@@ -177,10 +217,14 @@ void Labeller::flush(int64_t keep_frame) {
   const size_t n = nodes_.size();
   if (n == 0) {
     previous_ = 0;
+    for (auto &a : arenas_)
+      spare_arenas_.push_back(std::move(a.second));
     arenas_.clear();
     return;
   }
-  std::vector<uint32_t> label(n, kNone), root(n);
+  std::vector<uint32_t> &label = label_, &root = root_;
+  label.assign(n, kNone);
+  root.resize(n);
   uint32_t components = 0;
   for (size_t i = 0; i < n; i++) {
     const uint32_t r = find(static_cast<uint32_t>(i));
@@ -188,16 +232,18 @@ void Labeller::flush(int64_t keep_frame) {
       label[r] = components++;
     root[i] = label[r];
   }
-  std::vector<uint32_t> offset(components + 1, 0);
+  std::vector<uint32_t> &offset = offset_, &cursor = cursor_, &member = member_;
+  offset.assign(components + 1, 0);
   for (size_t i = 0; i < n; i++)
     offset[root[i] + 1]++;
   for (size_t c = 0; c < components; c++)
     offset[c + 1] += offset[c];
-  std::vector<uint32_t> cursor(offset.begin(), offset.end() - 1);
-  std::vector<uint32_t> member(n);
+  cursor.assign(offset.begin(), offset.end() - 1);
+  member.resize(n);
   for (size_t i = 0; i < n; i++)
     member[cursor[root[i]]++] = static_cast<uint32_t>(i);
-  std::vector<char> open(components, 0);
+  std::vector<char> &open = open_;
+  open.assign(components, 0);
   for (size_t i = 0; i < n; i++)
     if (nodes_[i].frame == keep_frame)
       open[root[i]] = 1;
@@ -205,7 +251,8 @@ void Labeller::flush(int64_t keep_frame) {
     if (!open[c])
       emit(&member[offset[c]], offset[c + 1] - offset[c]);
 
-  std::vector<uint32_t> first_kept(components, kNone);
+  std::vector<uint32_t> &first_kept = first_kept_;
+  first_kept.assign(components, kNone);
   size_t write = 0;
   live_pixels_ = 0;
   for (size_t i = 0; i < n; i++) {
@@ -229,7 +276,12 @@ void Labeller::flush(int64_t keep_frame) {
     bool used = false;
     for (size_t i = 0; i < write && !used; i++)
       used = nodes_[i].frame == it->first;
-    it = used ? std::next(it) : arenas_.erase(it);
+    if (used) {
+      ++it;
+    } else {
+      spare_arenas_.push_back(std::move(it->second));
+      it = arenas_.erase(it);
+    }
   }
 }
 
@@ -354,8 +406,10 @@ void Labeller::emit(const uint32_t *members, size_t count) {
   }
 
   // The spot's pixels in (frame, index) order: its components come in
-  // (frame, first) order, and those sharing a frame are merged by index.
+  // (frame, first) order, and those sharing a frame are merged by index.  A
+  // frame with one component of the spot (nearly always) is a straight copy.
   spot.first = kept_.size();
+  kept_.reserve(kept_.size() + n);
   size_t k = 0;
   while (k < count) {
     const int64_t z = nodes_[members[k]].frame;
@@ -363,7 +417,15 @@ void Labeller::emit(const uint32_t *members, size_t count) {
     while (e < count && nodes_[members[e]].frame == z)
       e++;
     const Arena &arena = arenas_.at(z);
-    std::vector<size_t> pos(e - k);
+    if (e == k + 1) {
+      const Node &node = nodes_[members[k]];
+      for (size_t i = node.begin; i < node.end; i++)
+        kept_.push_back(sf_pixel{static_cast<int32_t>(z), arena.index[i], arena.value[i]});
+      k = e;
+      continue;
+    }
+    pos_.assign(e - k, 0);
+    std::vector<size_t> &pos = pos_;
     for (size_t j = k; j < e; j++)
       pos[j - k] = nodes_[members[j]].begin;
     for (;;) {

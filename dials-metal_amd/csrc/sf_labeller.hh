@@ -6,12 +6,41 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <cstdlib>
 #include <map>
 #include <vector>
 
 #include "spotfinder_hip.h"
 
 namespace sfl {
+
+// A growable array of plain records in memory advised for huge pages.  A
+// sweep's spot pixels reach hundreds of megabytes: grown in 4 KB pages (and
+// copied at every doubling) that memory cost more than the grouping itself.
+template <typename T> class BigArray {
+public:
+  BigArray() = default;
+  BigArray(const BigArray &) = delete;
+  BigArray &operator=(const BigArray &) = delete;
+  ~BigArray() { std::free(data_); }
+  void push_back(const T &v) {
+    if (size_ == cap_)
+      grow(size_ + 1);
+    data_[size_++] = v;
+  }
+  void reserve(size_t n) {
+    if (n > cap_)
+      grow(n);
+  }
+  const T *data() const { return data_; }
+  size_t size() const { return size_; }
+  size_t capacity() const { return cap_; }
+
+private:
+  void grow(size_t need);
+  T *data_ = nullptr;
+  size_t size_ = 0, cap_ = 0;
+};
 
 // dials_spots::Options (src/spots/dials_spots.hh:81-95).
 struct Options {
@@ -41,8 +70,8 @@ public:
 
   bool started() const { return started_; }
   int64_t last_frame() const { return last_frame_; }
-  const std::vector<sf_spot> &spots() const { return spots_; }
-  const std::vector<sf_pixel> &pixels() const { return kept_; }
+  const BigArray<sf_spot> &spots() const { return spots_; }
+  const BigArray<sf_pixel> &pixels() const { return kept_; }
   const sf_counts &counts() const { return counts_; }
   size_t live() const { return live_pixels_; }
   size_t height() const { return height_; }
@@ -73,8 +102,15 @@ private:
   int64_t last_frame_ = 0;
   bool started_ = false;
   size_t live_pixels_ = 0;
-  std::vector<sf_spot> spots_;
-  std::vector<sf_pixel> kept_;
+  BigArray<sf_spot> spots_;
+  BigArray<sf_pixel> kept_;
+  std::vector<Arena> spare_arenas_; // released frames' buffers, for reuse
+  // Scratch reused from frame to frame: add's counting sort, flush's
+  // labels, members and marks, emit's merge cursors.
+  std::vector<uint32_t> start_, at_, label_, root_, offset_, cursor_, member_,
+      first_kept_;
+  std::vector<char> open_;
+  std::vector<size_t> pos_;
   sf_counts counts_{};
 };
 
