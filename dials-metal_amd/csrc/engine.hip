@@ -539,8 +539,12 @@ void Engine::label_counts(int id, size_t *n, size_t *ncomp, size_t *nlinks,
 
 void Engine::fetch_labels(int id, uint32_t *comp_of, size_t n,
                           sf_component *comps, size_t ncomp, uint32_t *links,
-                          size_t nlinks) {
+                          size_t nlinks, sf_signal_pixel *list) {
   FrameSlot &s = slot(id);
+  if (list && n)
+    check(hipMemcpyAsync(list, s.ordered.ptr, n * sizeof(sf_signal_pixel),
+                         hipMemcpyDeviceToHost, stream_),
+          "copying the signal pixels back");
   if (comp_of && n)
     check(hipMemcpyAsync(comp_of, s.comp_of.ptr, n * 4, hipMemcpyDeviceToHost,
                          stream_),

@@ -44,6 +44,14 @@ namespace {
 
 constexpr uint32_t kNone = 0xFFFFFFFFu;
 constexpr double kPixelVariance = 1.0 / 12.0;
+// An earlier frame's arena is released, its kept pixels moved, once fewer
+// than one pixel in kRetainRatio is still read; the retained store is
+// rewritten once it holds more than twice its live pixels plus kRetainSlack.
+constexpr size_t kRetainRatio = 8;
+constexpr size_t kRetainSlack = 1024;
+// Closed components' nodes are dropped once they outnumber the live ones by
+// kDeadSlack.
+constexpr size_t kDeadSlack = 1024;
 using u128 = unsigned __int128;
 
 u128 pair128(const uint64_t p[2]) {
@@ -134,10 +142,11 @@ void Labeller::merge(uint32_t a, uint32_t b) {
   const uint32_t ra = find(a), rb = find(b);
   if (ra == rb)
     return;
-  if (ra < rb)
-    parent_[rb] = ra;
-  else
-    parent_[ra] = rb;
+  // The lower node stays the root; its chain takes the other's.
+  const uint32_t keep = ra < rb ? ra : rb, join = ra < rb ? rb : ra;
+  parent_[join] = keep;
+  next_[tail_[keep]] = join;
+  tail_[keep] = tail_[join];
 }
 
 void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
@@ -185,9 +194,16 @@ void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
     node.end = start[c + 1];
     if (node.end - node.begin != node.c.n)
       throw std::runtime_error("device labelling: component sizes disagree");
+    const uint32_t self = static_cast<uint32_t>(nodes_.size());
     nodes_.push_back(node);
-    parent_.push_back(static_cast<uint32_t>(nodes_.size() - 1));
+    parent_.push_back(self);
+    next_.push_back(kNone);
+    tail_.push_back(self);
   }
+  arena.live = n;
+  arena.first = base;
+  arena.last = nodes_.size();
+  live_nodes_ += ncomp;
   counts_.signal_pixels += n;
   live_pixels_ += n;
 
@@ -214,75 +230,167 @@ void Labeller::finish() {
 }
 
 void Labeller::flush(int64_t keep_frame) {
+  // Every open component has a node on the last frame, so the nodes from
+  // there on name every component that can close now: those with no node on
+  // keep_frame.  Marks: 1 seen, 2 open.
   const size_t n = nodes_.size();
-  if (n == 0) {
+  mark_.resize(n, 0);
+  std::vector<uint32_t> &roots = roots_;
+  roots.clear();
+  for (size_t i = previous_; i < n; i++) {
+    if (nodes_[i].dead)
+      continue;
+    const uint32_t r = find(static_cast<uint32_t>(i));
+    if (!mark_[r]) {
+      mark_[r] = 1;
+      roots.push_back(r);
+    }
+    if (nodes_[i].frame == keep_frame)
+      mark_[r] = 2;
+  }
+  // Closing components in order of their first node, as the reference numbers
+  // them (dials_spots.cc:158-170); each one's nodes in (frame, first) order.
+  std::sort(roots.begin(), roots.end());
+  std::vector<uint32_t> &member = member_;
+  for (const uint32_t r : roots) {
+    const bool open = mark_[r] == 2;
+    mark_[r] = 0;
+    if (open)
+      continue;
+    member.clear();
+    for (uint32_t k = r; k != kNone; k = next_[k])
+      member.push_back(k);
+    if (!std::is_sorted(member.begin(), member.end()))
+      std::sort(member.begin(), member.end());
+    emit(member.data(), member.size());
+    for (const uint32_t k : member) {
+      Node &node = nodes_[k];
+      node.dead = true;
+      live_pixels_ -= node.c.n;
+      const size_t held = node.end - node.begin;
+      if (node.retained) {
+        retained_live_ -= held;
+      } else {
+        auto it = arenas_.find(node.frame);
+        it->second.live -= held;
+        if (it->second.live == 0)
+          release(it);
+      }
+    }
+    live_nodes_ -= member.size();
+  }
+
+  if (live_nodes_ == 0) {
+    nodes_.clear();
+    parent_.clear();
+    next_.clear();
+    tail_.clear();
+    mark_.clear();
     previous_ = 0;
     for (auto &a : arenas_)
       spare_arenas_.push_back(std::move(a.second));
     arenas_.clear();
+    retained_.index.clear();
+    retained_.value.clear();
+    retained_live_ = 0;
     return;
   }
-  std::vector<uint32_t> &label = label_, &root = root_;
-  label.assign(n, kNone);
-  root.resize(n);
-  uint32_t components = 0;
-  for (size_t i = 0; i < n; i++) {
-    const uint32_t r = find(static_cast<uint32_t>(i));
-    if (label[r] == kNone)
-      label[r] = components++;
-    root[i] = label[r];
-  }
-  std::vector<uint32_t> &offset = offset_, &cursor = cursor_, &member = member_;
-  offset.assign(components + 1, 0);
-  for (size_t i = 0; i < n; i++)
-    offset[root[i] + 1]++;
-  for (size_t c = 0; c < components; c++)
-    offset[c + 1] += offset[c];
-  cursor.assign(offset.begin(), offset.end() - 1);
-  member.resize(n);
-  for (size_t i = 0; i < n; i++)
-    member[cursor[root[i]]++] = static_cast<uint32_t>(i);
-  std::vector<char> &open = open_;
-  open.assign(components, 0);
-  for (size_t i = 0; i < n; i++)
-    if (nodes_[i].frame == keep_frame)
-      open[root[i]] = 1;
-  for (uint32_t c = 0; c < components; c++)
-    if (!open[c])
-      emit(&member[offset[c]], offset[c + 1] - offset[c]);
+  if (n - live_nodes_ > live_nodes_ + kDeadSlack)
+    compact();
+  previous_ = nodes_.size();
+  while (previous_ > 0 && nodes_[previous_ - 1].frame == keep_frame)
+    previous_--;
 
-  std::vector<uint32_t> &first_kept = first_kept_;
-  first_kept.assign(components, kNone);
-  size_t write = 0;
-  live_pixels_ = 0;
-  for (size_t i = 0; i < n; i++) {
-    const uint32_t c = root[i];
-    if (!open[c])
+  // An earlier frame's arena that only a few long-lived components still
+  // read (hot pixels and powder rings stay open for the whole sweep) hands
+  // their pixels to the retained store and is released, so memory follows
+  // the live pixels and not the sweep's length.  The retained store is
+  // rewritten once it is mostly closed components'.
+  for (auto it = arenas_.begin(); it != arenas_.end() && it->first < keep_frame;) {
+    Arena &arena = it->second;
+    if (arena.live > 0 && arena.live * kRetainRatio >= arena.index.size()) {
+      ++it;
       continue;
-    if (first_kept[c] == kNone)
-      first_kept[c] = static_cast<uint32_t>(write);
+    }
+    for (size_t i = arena.first; i < arena.last; i++)
+      if (!nodes_[i].dead) {
+        move_pixels(arena, nodes_[i], retained_);
+        retained_live_ += nodes_[i].end - nodes_[i].begin;
+      }
+    auto next = std::next(it);
+    release(it);
+    it = next;
+  }
+  if (retained_.index.size() > 2 * retained_live_ + kRetainSlack) {
+    Arena next;
+    next.index.reserve(2 * retained_live_ + kRetainSlack);
+    next.value.reserve(2 * retained_live_ + kRetainSlack);
+    for (Node &node : nodes_)
+      if (!node.dead && node.retained)
+        move_pixels(retained_, node, next);
+    std::swap(retained_.index, next.index);
+    std::swap(retained_.value, next.value);
+  }
+}
+void Labeller::release(std::map<int64_t, Arena>::iterator it) {
+  it->second.live = 0;
+  spare_arenas_.push_back(std::move(it->second));
+  arenas_.erase(it);
+}
+// Drops dead nodes, keeping order: roots stay each component's lowest node,
+// and the chains are rebuilt in node order.
+void Labeller::compact() {
+  const size_t n = nodes_.size();
+  std::vector<uint32_t> &remap = remap_, &root_of = roots_;
+  // Every root before any entry moves: find() reads the old parents.
+  root_of.assign(n, kNone);
+  for (size_t i = 0; i < n; i++)
+    if (!nodes_[i].dead)
+      root_of[i] = find(static_cast<uint32_t>(i));
+  remap.assign(n, kNone);
+  size_t write = 0;
+  for (size_t i = 0; i < n; i++) {
+    if (nodes_[i].dead)
+      continue;
+    remap[i] = static_cast<uint32_t>(write);
+    const uint32_t root = remap[root_of[i]]; // a root is its lowest node
     nodes_[write] = nodes_[i];
-    parent_[write] = first_kept[c];
-    live_pixels_ += nodes_[write].c.n;
+    parent_[write] = root;
+    next_[write] = kNone;
+    if (root == write)
+      tail_[write] = root;
+    else
+      next_[tail_[root]] = static_cast<uint32_t>(write), tail_[root] = static_cast<uint32_t>(write);
     write++;
   }
   nodes_.resize(write);
   parent_.resize(write);
-  previous_ = write;
-  while (previous_ > 0 && nodes_[previous_ - 1].frame == keep_frame)
-    previous_--;
-  // Pixels of frames that no open component touches are no longer needed.
-  for (auto it = arenas_.begin(); it != arenas_.end();) {
-    bool used = false;
-    for (size_t i = 0; i < write && !used; i++)
-      used = nodes_[i].frame == it->first;
-    if (used) {
-      ++it;
-    } else {
-      spare_arenas_.push_back(std::move(it->second));
-      it = arenas_.erase(it);
-    }
+  next_.resize(write);
+  tail_.resize(write);
+  mark_.assign(write, 0);
+  for (auto &a : arenas_) {
+    Arena &arena = a.second;
+    size_t first = kNone, last = 0;
+    for (size_t i = arena.first; i < arena.last; i++)
+      if (remap[i] != kNone) {
+        first = std::min<size_t>(first, remap[i]);
+        last = remap[i] + 1;
+      }
+    arena.first = first == kNone ? 0 : first;
+    arena.last = last;
   }
+}
+void Labeller::move_pixels(const Arena &from, Node &node, Arena &to) {
+  const size_t begin = to.index.size(), count = node.end - node.begin;
+  if (begin + count > 0xFFFFFFFFu)
+    throw std::runtime_error("dials_spots: more than 2^32 pixels held open");
+  to.index.insert(to.index.end(), from.index.begin() + node.begin,
+                  from.index.begin() + node.end);
+  to.value.insert(to.value.end(), from.value.begin() + node.begin,
+                  from.value.begin() + node.end);
+  node.begin = static_cast<uint32_t>(begin);
+  node.end = static_cast<uint32_t>(begin + count);
+  node.retained = true;
 }
 
 void Labeller::emit(const uint32_t *members, size_t count) {
@@ -416,7 +524,9 @@ void Labeller::emit(const uint32_t *members, size_t count) {
     size_t e = k;
     while (e < count && nodes_[members[e]].frame == z)
       e++;
-    const Arena &arena = arenas_.at(z);
+    // A frame's kept nodes move to the retained store together.
+    const Arena &arena =
+        nodes_[members[k]].retained ? retained_ : arenas_.at(z);
     if (e == k + 1) {
       const Node &node = nodes_[members[k]];
       for (size_t i = node.begin; i < node.end; i++)

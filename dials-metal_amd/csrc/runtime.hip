@@ -215,6 +215,36 @@ bool ascending_in_frame(const sf_signal_pixel *s, size_t n, size_t pixels) {
 
 } // namespace
 
+// Pinned host memory that only grows.
+class Pinned {
+public:
+  Pinned() = default;
+  Pinned(const Pinned &) = delete;
+  Pinned &operator=(const Pinned &) = delete;
+  ~Pinned() {
+    if (p_)
+      (void)hipHostFree(p_);
+  }
+  template <typename T> T *get(size_t count) {
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    if (bytes > cap_) {
+      const size_t next = std::max(bytes, 2 * cap_);
+      void *q = nullptr;
+      check(hipHostMalloc(&q, next, hipHostMallocDefault),
+            "hipHostMalloc (the grouping's copies)");
+      if (p_)
+        (void)hipHostFree(p_);
+      p_ = q;
+      cap_ = next;
+    }
+    return static_cast<T *>(p_);
+  }
+
+private:
+  void *p_ = nullptr;
+  size_t cap_ = 0;
+};
+
 // The grouping, with the engine state needed to honour frame links: a frame's
 // links count only if they were taken against the frame added just before it.
 struct sf_labeller {
@@ -223,32 +253,27 @@ struct sf_labeller {
   std::unique_ptr<sf_workspace> ws; // for host lists (sf_labeller_add)
   const Engine *last_engine = nullptr;
   uint64_t last_serial = 0;
-  std::vector<sf_signal_pixel> list;
-  std::vector<uint32_t> comp_of, links;
-  std::vector<sf_component> comps;
+  // A frame's lists come back into pinned memory, which the copy engines
+  // write at the link's rate (pageable memory goes through a bounce buffer).
+  Pinned list, comp_of, links, comps;
 
   // Adds engine slot `id` as `frame`; `host_list` when the caller has it.
+  // `fresh` false reuses the counters read for an earlier slot of the same
+  // labelled batch.
   void add_slot(int64_t frame, Engine &e, int id,
-                const sf_signal_pixel *host_list) {
+                const sf_signal_pixel *host_list, bool fresh = true) {
     size_t n = 0, ncomp = 0, nlinks = 0;
-    e.label_counts(id, &n, &ncomp, &nlinks);
-    const sf_signal_pixel *l = host_list;
-    if (!l) {
-      list.resize(n);
-      e.fetch(id, list.data(), n);
-      l = list.data();
-      lab.check_frame(frame, l, n);
-    }
-    comp_of.resize(n);
-    comps.resize(ncomp);
-    links.resize(2 * nlinks);
-    e.fetch_labels(id, comp_of.data(), n, comps.data(), ncomp, links.data(),
-                   nlinks);
+    e.label_counts(id, &n, &ncomp, &nlinks, fresh);
+    sf_signal_pixel *fetched = host_list ? nullptr : list.get<sf_signal_pixel>(n);
+    uint32_t *c_of = comp_of.get<uint32_t>(n);
+    sf_component *cs = comps.get<sf_component>(ncomp);
+    uint32_t *ln = links.get<uint32_t>(2 * nlinks);
+    e.fetch_labels(id, c_of, n, cs, ncomp, ln, nlinks, fetched);
+    const sf_signal_pixel *l = host_list ? host_list : fetched;
     const FrameSlot &s = e.slot(id);
     const bool to_previous = s.linked_serial != 0 && lab.started() &&
                              last_engine == &e && last_serial == s.linked_serial;
-    lab.add(frame, l, n, comp_of.data(), comps.data(), ncomp, links.data(),
-            nlinks, to_previous);
+    lab.add(frame, l, n, c_of, cs, ncomp, ln, nlinks, to_previous);
     last_engine = &e;
     last_serial = s.serial;
   }
@@ -906,7 +931,7 @@ int sf_labeller_add_batch(sf_labeller *l, int64_t first_frame, sf_batch *b) {
   return add_checked(l, [&] {
     for (size_t i = 0; i < b->ids.size(); i++)
       l->add_slot(first_frame + static_cast<int64_t>(i), b->engine, b->ids[i],
-                  nullptr);
+                  nullptr, i == 0);
   });
 }
 

@@ -82,22 +82,35 @@ private:
   struct Node {
     int64_t frame;
     sf_component c;
-    uint32_t begin, end; // pixel range in the frame's arena
+    uint32_t begin, end;   // pixel range in the frame's arena
+    bool retained = false; // the range is in retained_ instead
+    bool dead = false;     // its component was emitted or dropped
   };
   struct Arena {
     std::vector<uint32_t> index, value;
+    size_t live = 0;             // pixels of live nodes still held here
+    size_t first = 0, last = 0;  // the frame's nodes, [first, last)
   };
 
   uint32_t find(uint32_t i);
   void merge(uint32_t a, uint32_t b);
   void flush(int64_t keep_frame);
   void emit(const uint32_t *members, size_t count);
+  static void move_pixels(const Arena &from, Node &node, Arena &to);
+  void release(std::map<int64_t, Arena>::iterator it);
+  void compact();
 
   size_t height_, width_;
   Options options_;
-  std::vector<Node> nodes_;      // open components, (frame, first) order
-  std::vector<uint32_t> parent_;
+  // Nodes in (frame, first index) order; closed ones stay, marked dead, until
+  // they outnumber the live ones.  Union-find keeps each component's lowest
+  // node as its root; next_ chains a component's nodes from its root, and
+  // tail_ (at the root) ends the chain.
+  std::vector<Node> nodes_;
+  std::vector<uint32_t> parent_, next_, tail_;
+  size_t live_nodes_ = 0;
   size_t previous_ = 0;          // where the last frame's nodes start
+  size_t retained_live_ = 0;     // live pixels in retained_
   std::map<int64_t, Arena> arenas_;
   int64_t last_frame_ = 0;
   bool started_ = false;
@@ -105,11 +118,12 @@ private:
   BigArray<sf_spot> spots_;
   BigArray<sf_pixel> kept_;
   std::vector<Arena> spare_arenas_; // released frames' buffers, for reuse
+  // Pixels of long-lived components whose frames' arenas were released.
+  Arena retained_;
   // Scratch reused from frame to frame: add's counting sort, flush's
   // labels, members and marks, emit's merge cursors.
-  std::vector<uint32_t> start_, at_, label_, root_, offset_, cursor_, member_,
-      first_kept_;
-  std::vector<char> open_;
+  std::vector<uint32_t> start_, at_, roots_, member_, remap_;
+  std::vector<uint8_t> mark_;
   std::vector<size_t> pos_;
   sf_counts counts_{};
 };

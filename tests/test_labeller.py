@@ -97,3 +97,48 @@ def test_bad_grouping_options(sf):
         sf.Labeller(8, 10, min_spot_size=5, max_spot_size=4)
     with pytest.raises(sf.LabellerError):
         sf.Labeller(8, 10, min_spot_size=0)
+
+
+def long_lived_sweep(frames=160, h=128, w=128, hot=10, warm=600, transient=2400,
+                     seed=5):
+    """Frames whose components outlive their frames' pixel buffers: `hot`
+    pixels on in every frame, `warm` pixels each on for a run of 5 to 15
+    frames, and `transient` pixels in alternating halves of the frame, which
+    never touch the next frame's."""
+    import spotfinder_hip as sf
+    rng = np.random.default_rng(seed)
+    band = np.arange(h * w).reshape(h, w)
+    hot_at = rng.choice(band[:, :16].ravel(), hot, replace=False)
+    warm_at = rng.choice(band[:, 16:32].ravel(), warm, replace=False)
+    start = rng.integers(0, frames, warm)
+    stop = start + rng.integers(5, 16, warm)
+    for z in range(frames):
+        half = band[(z % 2) * (h // 2):(z % 2 + 1) * (h // 2), 32:].ravel()
+        idx = np.concatenate([hot_at, warm_at[(start <= z) & (z < stop)],
+                              rng.choice(half, transient, replace=False)])
+        idx.sort()
+        px = np.zeros(len(idx), sf.SIGNAL_PIXEL)
+        px["index"] = idx
+        px["value"] = rng.integers(5, 60, len(idx))
+        yield z, px
+
+
+def test_host_grouping_long_lived_components(sf):
+    """Components open across most of the sweep: their frames' pixel buffers
+    are released early and their pixels held apart, and that store is
+    rewritten as the warm pixels' components close. The spots, their pixels
+    and the counts equal the oracle's labeller."""
+    kw = dict(max_spot_size=100000, max_separation=1000.0)
+    ours = ModelLabeller(128, 128, **kw)
+    want = O.oracle().labeller(128, 128, **kw)
+    for z, px in long_lived_sweep():
+        ours.add(z, px)
+        want.add(z, px)
+    ours.finish()
+    want.finish()
+    got = ours.spots()
+    assert len(got) > 1000
+    assert (got["n_signal"] == 160).sum() == 10  # the hot pixels, whole sweep
+    compare_spots(got, want.spots())
+    assert ours.pixels().tobytes() == want.pixels().tobytes()
+    assert ours.counts() == want.counts()
