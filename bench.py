@@ -312,16 +312,24 @@ def narrow(sf, frames_host, per_step=32, steps=10):
 
 
 def pipeline_3d(sf, dev, h, w, batches=4):
-    """The whole spot-finding chain in 3D: `batches` steps of the benchmark's
-    32 resident frames (threshold, ordering and labelling on the device), each
-    batch's components stitched across frames into spots by the host grouping
-    (sf_labeller_add_batch), as consecutive frames of one sweep."""
+    """The whole spot-finding chain in 3D, twice: with the grouping's pixels
+    on the host, and with them kept on the device (the host stitch then sees
+    components only)."""
+    return {kind: pipeline_3d_run(sf, dev, h, w, batches, kind == "device_pixels")
+            for kind in ("host_pixels", "device_pixels")}
+
+
+def pipeline_3d_run(sf, dev, h, w, batches, device_pixels):
+    """`batches` steps of the benchmark's 32 resident frames (threshold,
+    ordering and labelling on the device), each batch's components stitched
+    across frames into spots by the grouping (sf_labeller_add_batch), as
+    consecutive frames of one sweep."""
     descs = [(d.ptr, h, w, 32) for d in dev]
     b = sf.Batch()
     b.threshold(descs, graph=True)
     b.label(True)
     b.sync()
-    lab = sf.Labeller(h, w)
+    lab = sf.Labeller(h, w, device_pixels=device_pixels)
     t_dev = t_host = 0.0
     for step in range(batches):
         t0 = time.perf_counter()
@@ -342,7 +350,8 @@ def pipeline_3d(sf, dev, h, w, batches=4):
             "value": n / (t_dev + t_host), "unit": "frames/s",
             "device_ms_per_frame": 1e3 * t_dev / n,
             "host_grouping_ms_per_frame": 1e3 * t_host / n,
-            "spots": len(lab.spots()), "counts": lab.counts()}
+            "spots": len(lab.spots()), "pixels": len(lab.pixels()),
+            "counts": lab.counts()}
 
 
 def refl(sf, repeat=64):

@@ -80,11 +80,12 @@ public:
                     bool fresh = true);
   // Device bytes a slot of this shape holds once labelled.
   static size_t slot_bytes(uint32_t height, uint32_t width, int bits);
-  // Waits; copies a labelled slot's labels, components, frame links and,
-  // when `list` is given, its signal pixels (n of each) with one wait.
+  // Copies a labelled slot's labels, components, frame links and, when
+  // `list` is given, its signal pixels (n of each), and waits unless `wait`
+  // is false (the caller then waits on stream()).
   void fetch_labels(int id, uint32_t *comp_of, size_t n, sf_component *comps,
                     size_t ncomp, uint32_t *links, size_t nlinks,
-                    sf_signal_pixel *list = nullptr);
+                    sf_signal_pixel *list = nullptr, bool wait = true);
 
   // Events around the threshold kernel(s) and the ordering of the last batch.
   void set_timing(bool on) { timing_on_ = on; }

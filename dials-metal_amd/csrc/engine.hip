@@ -539,7 +539,7 @@ void Engine::label_counts(int id, size_t *n, size_t *ncomp, size_t *nlinks,
 
 void Engine::fetch_labels(int id, uint32_t *comp_of, size_t n,
                           sf_component *comps, size_t ncomp, uint32_t *links,
-                          size_t nlinks, sf_signal_pixel *list) {
+                          size_t nlinks, sf_signal_pixel *list, bool wait) {
   FrameSlot &s = slot(id);
   if (list && n)
     check(hipMemcpyAsync(list, s.ordered.ptr, n * sizeof(sf_signal_pixel),
@@ -557,7 +557,8 @@ void Engine::fetch_labels(int id, uint32_t *comp_of, size_t n,
     check(hipMemcpyAsync(links, s.links.ptr, nlinks * 8, hipMemcpyDeviceToHost,
                          stream_),
           "copying the frame links back");
-  check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  if (wait)
+    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
 }
 
 void Engine::kernel_ms(double *threshold_ms, double *order_ms) {

@@ -153,9 +153,10 @@ void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
                    const uint32_t *comp_of, const sf_component *comps,
                    size_t ncomp, const uint32_t *links, size_t nlinks,
                    bool links_to_previous) {
+  if (device_)
+    throw std::runtime_error("dials_spots: this labeller takes device frames");
   check_frame(frame, signal, n);
-  if (started_ && frame != last_frame_ + 1)
-    flush(frame);
+  begin_frame(frame);
 
   // Pixels of this frame grouped by component, each group still ascending.
   Arena &arena = arenas_[frame];
@@ -183,7 +184,49 @@ void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
     arena.index[k] = signal[i].index;
     arena.value[k] = signal[i].value;
   }
-
+  add_nodes(frame, arena, start, n, comps, ncomp, links, nlinks,
+            links_to_previous);
+}
+void Labeller::use_device_pixels() {
+  if (started_)
+    throw std::runtime_error("dials_spots: device mode is chosen before the "
+                             "first frame");
+  device_ = true;
+}
+void Labeller::add_device(int64_t frame, const void *pixels, uint64_t owner,
+                          size_t n, const sf_component *comps, size_t ncomp,
+                          const uint32_t *links, size_t nlinks,
+                          bool links_to_previous) {
+  if (!device_)
+    throw std::runtime_error("dials_spots: this labeller takes host frames");
+  if (started_ && frame <= last_frame_)
+    throw std::runtime_error("dials_spots: frame " + std::to_string(frame) +
+                             " arrived after frame " +
+                             std::to_string(last_frame_) +
+                             "; frames must be added in ascending order");
+  // The components' sizes place them in the frame's grouped pixels.
+  std::vector<uint32_t> &start = start_;
+  start.assign(ncomp + 1, 0);
+  for (size_t c = 0; c < ncomp; c++)
+    start[c + 1] = start[c] + comps[c].n;
+  if (start[ncomp] != n)
+    throw std::runtime_error("device labelling: component sizes disagree");
+  begin_frame(frame);
+  Arena &arena = arenas_[frame];
+  arena.device = pixels;
+  arena.owner = owner;
+  add_nodes(frame, arena, start, n, comps, ncomp, links, nlinks,
+            links_to_previous);
+}
+void Labeller::begin_frame(int64_t frame) {
+  if (started_ && frame != last_frame_ + 1)
+    flush(frame);
+}
+void Labeller::add_nodes(int64_t frame, Arena &arena,
+                         const std::vector<uint32_t> &start, size_t n,
+                         const sf_component *comps, size_t ncomp,
+                         const uint32_t *links, size_t nlinks,
+                         bool links_to_previous) {
   const size_t base = nodes_.size();
   const size_t previous = previous_;
   for (size_t c = 0; c < ncomp; c++) {
@@ -287,9 +330,8 @@ void Labeller::flush(int64_t keep_frame) {
     tail_.clear();
     mark_.clear();
     previous_ = 0;
-    for (auto &a : arenas_)
-      spare_arenas_.push_back(std::move(a.second));
-    arenas_.clear();
+    while (!arenas_.empty())
+      release(arenas_.begin());
     retained_.index.clear();
     retained_.value.clear();
     retained_live_ = 0;
@@ -308,7 +350,8 @@ void Labeller::flush(int64_t keep_frame) {
   // rewritten once it is mostly closed components'.
   for (auto it = arenas_.begin(); it != arenas_.end() && it->first < keep_frame;) {
     Arena &arena = it->second;
-    if (arena.live > 0 && arena.live * kRetainRatio >= arena.index.size()) {
+    if (arena.live > 0 &&
+        (device_ || arena.live * kRetainRatio >= arena.index.size())) {
       ++it;
       continue;
     }
@@ -333,8 +376,12 @@ void Labeller::flush(int64_t keep_frame) {
   }
 }
 void Labeller::release(std::map<int64_t, Arena>::iterator it) {
-  it->second.live = 0;
-  spare_arenas_.push_back(std::move(it->second));
+  if (device_) {
+    released_.push_back(it->second.owner);
+  } else {
+    it->second.live = 0;
+    spare_arenas_.push_back(std::move(it->second));
+  }
   arenas_.erase(it);
 }
 // Drops dead nodes, keeping order: roots stay each component's lowest node,
@@ -516,6 +563,35 @@ void Labeller::emit(const uint32_t *members, size_t count) {
   // The spot's pixels in (frame, index) order: its components come in
   // (frame, first) order, and those sharing a frame are merged by index.  A
   // frame with one component of the spot (nearly always) is a straight copy.
+  if (device_) {
+    // The spot's pixels are written on the device: a task per node, the
+    // nodes sharing a frame forming one group merged by index there.
+    spot.first = kept_total_;
+    uint64_t out = kept_total_;
+    size_t k = 0;
+    while (k < count) {
+      const int64_t z = nodes_[members[k]].frame;
+      size_t e = k;
+      while (e < count && nodes_[members[e]].frame == z)
+        e++;
+      const Arena &arena = arenas_.at(z);
+      const uint32_t group = static_cast<uint32_t>(tasks_.size());
+      uint64_t held = 0;
+      for (size_t j = k; j < e; j++) {
+        const Node &node = nodes_[members[j]];
+        tasks_.push_back(GatherTask{arena.device, node.begin, node.end, out,
+                                    static_cast<int32_t>(z), group,
+                                    static_cast<uint32_t>(e - k), 0});
+        held += node.end - node.begin;
+      }
+      out += held;
+      k = e;
+    }
+    kept_total_ = out;
+    spots_.push_back(spot);
+    counts_.accepted++;
+    return;
+  }
   spot.first = kept_.size();
   kept_.reserve(kept_.size() + n);
   size_t k = 0;

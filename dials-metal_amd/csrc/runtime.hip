@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "engine.hh"
+#include "group_device.hh"
 #include "sf_internal.hh"
 #include "sf_labeller.hh"
 #include "spotfinder_hip.h"
@@ -137,13 +138,13 @@ struct sf_workspace {
   bool has_frame() { return engine.slot_count() > 0 && slot().thresholded; }
 };
 
-// A batch: an engine whose slots form two sets of frames_per_launch, one set
-// per batch in turn.
+// A batch: an engine whose slots hold each batch in a contiguous range that
+// leaves the batch before it untouched (its last frame links the next batch's
+// first): from slot 0 when the batch fits below the previous one's range,
+// else just above it.
 struct sf_batch {
   explicit sf_batch(void *stream) : engine(stream) {}
   Engine engine;
-  size_t set_size = 0;
-  int set = 1;
   std::vector<int> ids; // slots of the last batch, in frame order
   int last_prev_id = -1; // the previous batch's last slot
 };
@@ -215,36 +216,6 @@ bool ascending_in_frame(const sf_signal_pixel *s, size_t n, size_t pixels) {
 
 } // namespace
 
-// Pinned host memory that only grows.
-class Pinned {
-public:
-  Pinned() = default;
-  Pinned(const Pinned &) = delete;
-  Pinned &operator=(const Pinned &) = delete;
-  ~Pinned() {
-    if (p_)
-      (void)hipHostFree(p_);
-  }
-  template <typename T> T *get(size_t count) {
-    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
-    if (bytes > cap_) {
-      const size_t next = std::max(bytes, 2 * cap_);
-      void *q = nullptr;
-      check(hipHostMalloc(&q, next, hipHostMallocDefault),
-            "hipHostMalloc (the grouping's copies)");
-      if (p_)
-        (void)hipHostFree(p_);
-      p_ = q;
-      cap_ = next;
-    }
-    return static_cast<T *>(p_);
-  }
-
-private:
-  void *p_ = nullptr;
-  size_t cap_ = 0;
-};
-
 // The grouping, with the engine state needed to honour frame links: a frame's
 // links count only if they were taken against the frame added just before it.
 struct sf_labeller {
@@ -262,6 +233,10 @@ struct sf_labeller {
   // labelled batch.
   void add_slot(int64_t frame, Engine &e, int id,
                 const sf_signal_pixel *host_list, bool fresh = true) {
+    if (pixels_on_device) {
+      add_device(frame, e, &id, 1, fresh);
+      return;
+    }
     size_t n = 0, ncomp = 0, nlinks = 0;
     e.label_counts(id, &n, &ncomp, &nlinks, fresh);
     sf_signal_pixel *fetched = host_list ? nullptr : list.get<sf_signal_pixel>(n);
@@ -276,6 +251,74 @@ struct sf_labeller {
     lab.add(frame, l, n, c_of, cs, ncomp, ln, nlinks, to_previous);
     last_engine = &e;
     last_serial = s.serial;
+  }
+
+  // Device mode: the pixels stay on the device (DevicePixels); only the
+  // components and links come back.  Slots ids[0..count) become frames
+  // first_frame onward.
+  std::unique_ptr<DevicePixels> dp;
+  bool pixels_on_device = false;
+  std::vector<size_t> n_, ncomp_, nlinks_, at_comp_, at_link_;
+  std::vector<const void *> frame_pixels_;
+
+  void add_device(int64_t first_frame, Engine &e, const int *ids, size_t count,
+                  bool fresh) {
+    n_.resize(count);
+    ncomp_.resize(count);
+    nlinks_.resize(count);
+    at_comp_.resize(count + 1);
+    at_link_.resize(count + 1);
+    at_comp_[0] = at_link_[0] = 0;
+    for (size_t i = 0; i < count; i++) {
+      e.label_counts(ids[i], &n_[i], &ncomp_[i], &nlinks_[i], fresh && i == 0);
+      at_comp_[i + 1] = at_comp_[i] + ncomp_[i];
+      at_link_[i + 1] = at_link_[i] + nlinks_[i];
+    }
+    sf_component *cs = comps.get<sf_component>(at_comp_[count]);
+    uint32_t *ln = links.get<uint32_t>(2 * at_link_[count]);
+    for (size_t i = 0; i < count; i++)
+      e.fetch_labels(ids[i], nullptr, 0, cs + at_comp_[i], ncomp_[i],
+                     ln + 2 * at_link_[i], nlinks_[i], nullptr, false);
+    uint64_t owner = 0;
+    frame_pixels_.resize(count);
+    dp->stage(e, ids, count, n_.data(), ncomp_.data(), &owner,
+              frame_pixels_.data());
+    check(hipStreamSynchronize(e.stream()), "hipStreamSynchronize");
+    size_t added = 0;
+    try {
+      for (size_t i = 0; i < count; i++) {
+        const FrameSlot &s = e.slot(ids[i]);
+        const bool to_previous = s.linked_serial != 0 && lab.started() &&
+                                 last_engine == &e &&
+                                 last_serial == s.linked_serial;
+        lab.add_device(first_frame + static_cast<int64_t>(i), frame_pixels_[i],
+                       owner, n_[i], cs + at_comp_[i], ncomp_[i],
+                       ln + 2 * at_link_[i], nlinks_[i], to_previous);
+        added++;
+        last_engine = &e;
+        last_serial = s.serial;
+      }
+    } catch (...) {
+      // Frames the stitch never took are not its to release.
+      for (size_t i = added; i < count; i++)
+        dp->release(owner);
+      throw;
+    }
+    drain();
+  }
+
+  // Writes the spots emitted so far into the pixel list and returns the
+  // frames the stitch released to the pool.
+  void drain() {
+    const size_t have = lab.pixels().size(), total = lab.kept_total();
+    if (total > have) {
+      sf_pixel *dest = lab.pixels_mut().extend(total - have);
+      dp->gather(lab.tasks(), have, total - have, dest);
+    }
+    lab.tasks().clear();
+    for (const uint64_t owner : lab.released())
+      dp->release(owner);
+    lab.released().clear();
   }
 };
 
@@ -619,16 +662,14 @@ int sf_batch_threshold(sf_batch *b, const sf_frame_desc *frames, size_t n,
         (frames[i].bits != 16 && frames[i].bits != 32))
       return bad("a frame's pointer, shape or bits");
   return guarded([&]() -> int {
-    // Two sets of slots in turn, each as large as the largest batch so far:
-    // the batch just before is never overwritten by this one.
-    if (n > b->set_size)
-      b->set_size = n;
     const int prev_last = b->ids.empty() ? -1 : b->ids.back();
-    b->set ^= 1;
+    const size_t prev_first = b->ids.empty() ? 0 : static_cast<size_t>(b->ids[0]);
+    const size_t first =
+        b->ids.empty() || n <= prev_first ? 0 : static_cast<size_t>(prev_last) + 1;
     b->ids.resize(n);
     std::vector<FrameIn> in(n);
     for (size_t i = 0; i < n; i++) {
-      b->ids[i] = static_cast<int>(static_cast<size_t>(b->set) * b->set_size + i);
+      b->ids[i] = static_cast<int>(first + i);
       in[i] = FrameIn{frames[i].frame, frames[i].height, frames[i].width,
                       frames[i].bits};
     }
@@ -864,6 +905,25 @@ int sf_labeller_create(size_t height, size_t width, const sf_grouping *g,
   }
 }
 
+int sf_labeller_create_device(size_t height, size_t width,
+                              const sf_grouping *g, sf_labeller **out) {
+  const int rc = sf_labeller_create(height, width, g, out);
+  if (rc != 0)
+    return rc;
+  return guarded([&]() -> int {
+    try {
+      (*out)->lab.use_device_pixels();
+      (*out)->dp = std::make_unique<DevicePixels>();
+      (*out)->pixels_on_device = true;
+      return 0;
+    } catch (...) {
+      delete *out;
+      *out = nullptr;
+      throw;
+    }
+  });
+}
+
 void sf_labeller_destroy(sf_labeller *l) { delete l; }
 
 int sf_labeller_add(sf_labeller *l, int64_t frame,
@@ -929,6 +989,10 @@ int sf_labeller_add_batch(sf_labeller *l, int64_t first_frame, sf_batch *b) {
                  "labeller's shape");
   }
   return add_checked(l, [&] {
+    if (l->pixels_on_device) {
+      l->add_device(first_frame, b->engine, b->ids.data(), b->ids.size(), true);
+      return;
+    }
     for (size_t i = 0; i < b->ids.size(); i++)
       l->add_slot(first_frame + static_cast<int64_t>(i), b->engine, b->ids[i],
                   nullptr, i == 0);
@@ -954,8 +1018,11 @@ int sf_labeller_add_labelled(sf_labeller *l, int64_t frame,
 int sf_labeller_finish(sf_labeller *l) {
   if (!l)
     return bad("labeller");
-  l->lab.finish();
-  return 0;
+  return add_checked(l, [&] {
+    l->lab.finish();
+    if (l->pixels_on_device)
+      l->drain();
+  });
 }
 
 size_t sf_labeller_spot_count(const sf_labeller *l) {

@@ -4,6 +4,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <cstddef>
 #include <cstdint>
 #include <stdexcept>
@@ -33,6 +34,36 @@ Tuning tuning();
 
 // A grow-only device allocation.  Growing waits for the stream first: work in
 // flight may still be reading the old buffer (dext_cuda.cu:506-525).
+// Pinned host memory that only grows.
+class Pinned {
+public:
+  Pinned() = default;
+  Pinned(const Pinned &) = delete;
+  Pinned &operator=(const Pinned &) = delete;
+  ~Pinned() {
+    if (p_)
+      (void)hipHostFree(p_);
+  }
+  template <typename T> T *get(size_t count) {
+    const size_t bytes = std::max<size_t>(count * sizeof(T), 256);
+    if (bytes > cap_) {
+      const size_t next = std::max(bytes, 2 * cap_);
+      void *q = nullptr;
+      check(hipHostMalloc(&q, next, hipHostMallocDefault),
+            "hipHostMalloc (pinned staging)");
+      if (p_)
+        (void)hipHostFree(p_);
+      p_ = q;
+      cap_ = next;
+    }
+    return static_cast<T *>(p_);
+  }
+
+private:
+  void *p_ = nullptr;
+  size_t cap_ = 0;
+};
+
 struct DeviceBuffer {
   void *ptr = nullptr;
   size_t bytes = 0;

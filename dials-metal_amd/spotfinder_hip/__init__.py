@@ -232,6 +232,8 @@ def lib():
         "sf_grouping_default": ([ctypes.POINTER(Grouping)], None),
         "sf_labeller_create": ([sz, sz, ctypes.POINTER(Grouping),
                                 ctypes.POINTER(vp)], ctypes.c_int),
+        "sf_labeller_create_device": ([sz, sz, ctypes.POINTER(Grouping),
+                                       ctypes.POINTER(vp)], ctypes.c_int),
         "sf_labeller_destroy": ([vp], None),
         "sf_labeller_add": ([vp, ctypes.c_int64, vp, sz], ctypes.c_int),
         "sf_labeller_add_ws": ([vp, ctypes.c_int64, vp], ctypes.c_int),
@@ -595,18 +597,25 @@ class LabellerError(RuntimeError):
 
 class Labeller:
     """dials_spots::Labeller (src/spots/dials_spots.hh:108-158), with the
-    labelling and per-spot reductions on the device."""
+    labelling and per-spot reductions on the device.  With `device_pixels`
+    the frames' pixels also stay on the device: the host stitch sees
+    components only, and the spots' pixel lists are written on the device
+    (sf_labeller_create_device); it then takes device frames and host lists
+    (add), not add_labelled."""
 
     def __init__(self, height, width, min_spot_size=3, max_spot_size=1000,
-                 max_separation=2.0, two_d=False):
+                 max_separation=2.0, two_d=False, device_pixels=False):
         g = Grouping()
         lib().sf_grouping_default(ctypes.byref(g))
         g.min_spot_size, g.max_spot_size = min_spot_size, max_spot_size
         g.max_separation, g.two_d = max_separation, int(bool(two_d))
         self.h = ctypes.c_void_p()
-        if lib().sf_labeller_create(height, width, ctypes.byref(g),
-                                    ctypes.byref(self.h)) != 0:
+        create = (lib().sf_labeller_create_device if device_pixels
+                  else lib().sf_labeller_create)
+        rc = create(height, width, ctypes.byref(g), ctypes.byref(self.h))
+        if rc == -1:
             raise LabellerError(last_error())
+        _check(rc)
 
     def __del__(self):
         try:

@@ -300,6 +300,17 @@ void sf_grouping_default(sf_grouping *g);
 typedef struct sf_labeller sf_labeller;
 int sf_labeller_create(size_t height, size_t width, const sf_grouping *g,
                        sf_labeller **out);
+/* The same grouping (dials_spots::Labeller, src/spots/dials_spots.hh:108-158)
+ * with the pixels kept on the device: each added frame's signal pixels are
+ * regrouped by component in device memory, only components and frame links
+ * come back to the host stitch, and the spots' pixel lists are written on
+ * the device and copied back as spots close.  Spots, pixels and counts are
+ * identical to sf_labeller_create's.  It takes frames from workspaces,
+ * batches and sf_labeller_add (which labels on the device); not
+ * sf_labeller_add_labelled.  Device memory held: the staged batches whose
+ * frames an open component still touches. */
+int sf_labeller_create_device(size_t height, size_t width,
+                              const sf_grouping *g, sf_labeller **out);
 void sf_labeller_destroy(sf_labeller *l);
 int sf_labeller_add(sf_labeller *l, int64_t frame,
                     const sf_signal_pixel *signal, size_t n);
