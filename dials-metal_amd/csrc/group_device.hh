@@ -50,7 +50,7 @@ private:
   std::vector<std::unique_ptr<DeviceBuffer>> spare_;
   uint64_t next_owner_ = 1;
   DeviceBuffer keys_, keys_sorted_, packed_, sort_temp_, descs_, tasks_, out_;
-  Pinned host_descs_, host_tasks_;
+  Pinned host_descs_, host_tasks_, host_out_;
 };
 
 } // namespace sfi

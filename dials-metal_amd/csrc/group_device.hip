@@ -238,10 +238,13 @@ void DevicePixels::gather(const std::vector<sfl::GatherTask> &tasks,
                      static_cast<uint32_t>(ntasks), base,
                      static_cast<sf_pixel *>(out_.ptr));
   check(hipGetLastError(), "launching gather_spots");
-  check(hipMemcpyAsync(dest, out_.ptr, count * sizeof(sf_pixel),
+  // Through pinned memory, which the copy engines write at the link's rate.
+  sf_pixel *staged = host_out_.get<sf_pixel>(count);
+  check(hipMemcpyAsync(staged, out_.ptr, count * sizeof(sf_pixel),
                        hipMemcpyDeviceToHost, stream_),
         "copying the spot pixels back");
   check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  std::memcpy(dest, staged, count * sizeof(sf_pixel));
 }
 
 void DevicePixels::release(uint64_t owner) {
