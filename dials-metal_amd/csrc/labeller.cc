@@ -58,6 +58,16 @@ u128 pair128(const uint64_t p[2]) {
   return (static_cast<u128>(p[1]) << 64) | p[0];
 }
 
+// a * b, exactly, or false when it overflows 128 bits.  Operands below 2^64,
+// the usual case, multiply without the overflow test.
+bool mul_exact(u128 a, u128 b, u128 *out) {
+  if ((a >> 64) == 0 && (b >> 64) == 0) {
+    *out = static_cast<u128>(static_cast<uint64_t>(a)) * static_cast<uint64_t>(b);
+    return true;
+  }
+  return !__builtin_mul_overflow(a, b, out);
+}
+
 double to_double(u128 v) {
   // Correctly rounded: split so that each part converts exactly, then one
   // rounding in the addition when the value is below 2^106.
@@ -232,13 +242,13 @@ void Labeller::add_nodes(int64_t frame, Arena &arena,
   for (size_t c = 0; c < ncomp; c++) {
     Node node;
     node.frame = frame;
-    node.c = comps[c];
     node.begin = start[c];
     node.end = start[c + 1];
-    if (node.end - node.begin != node.c.n)
+    if (node.end - node.begin != comps[c].n)
       throw std::runtime_error("device labelling: component sizes disagree");
     const uint32_t self = static_cast<uint32_t>(nodes_.size());
     nodes_.push_back(node);
+    comps_.push_back(comps[c]);
     parent_.push_back(self);
     next_.push_back(kNone);
     tail_.push_back(self);
@@ -293,13 +303,16 @@ void Labeller::flush(int64_t keep_frame) {
   }
   // Closing components in order of their first node, as the reference numbers
   // them (dials_spots.cc:158-170); each one's nodes in (frame, first) order.
+  size_t closing = 0;
+  for (const uint32_t r : roots) {
+    if (mark_[r] != 2)
+      roots[closing++] = r;
+    mark_[r] = 0;
+  }
+  roots.resize(closing);
   std::sort(roots.begin(), roots.end());
   std::vector<uint32_t> &member = member_;
   for (const uint32_t r : roots) {
-    const bool open = mark_[r] == 2;
-    mark_[r] = 0;
-    if (open)
-      continue;
     member.clear();
     for (uint32_t k = r; k != kNone; k = next_[k])
       member.push_back(k);
@@ -309,7 +322,7 @@ void Labeller::flush(int64_t keep_frame) {
     for (const uint32_t k : member) {
       Node &node = nodes_[k];
       node.dead = true;
-      live_pixels_ -= node.c.n;
+      live_pixels_ -= comps_[k].n;
       const size_t held = node.end - node.begin;
       if (node.retained) {
         retained_live_ -= held;
@@ -325,6 +338,7 @@ void Labeller::flush(int64_t keep_frame) {
 
   if (live_nodes_ == 0) {
     nodes_.clear();
+    comps_.clear();
     parent_.clear();
     next_.clear();
     tail_.clear();
@@ -402,6 +416,7 @@ void Labeller::compact() {
     remap[i] = static_cast<uint32_t>(write);
     const uint32_t root = remap[root_of[i]]; // a root is its lowest node
     nodes_[write] = nodes_[i];
+    comps_[write] = comps_[i];
     parent_[write] = root;
     next_[write] = kNone;
     if (root == write)
@@ -411,6 +426,7 @@ void Labeller::compact() {
     write++;
   }
   nodes_.resize(write);
+  comps_.resize(write);
   parent_.resize(write);
   next_.resize(write);
   tail_.resize(write);
@@ -444,7 +460,7 @@ void Labeller::emit(const uint32_t *members, size_t count) {
   counts_.groups++;
   uint64_t n = 0;
   for (size_t k = 0; k < count; k++)
-    n += nodes_[members[k]].c.n;
+    n += comps_[members[k]].n;
   if (n < options_.min_spot_size) {
     counts_.too_small++;
     return;
@@ -465,7 +481,7 @@ void Labeller::emit(const uint32_t *members, size_t count) {
   bool have_peak = false;
   for (size_t k = 0; k < count; k++) {
     const Node &node = nodes_[members[k]];
-    const sf_component &c = node.c;
+    const sf_component &c = comps_[members[k]];
     const int64_t z = node.frame;
     if (k == 0) {
       x0 = c.x0, x1 = c.x1, y0 = c.y0, y1 = c.y1, z0 = z, z1 = z + 1;
@@ -527,8 +543,7 @@ void Labeller::emit(const uint32_t *members, size_t count) {
         // and W at doubled centres.
         double delta_sq;
         u128 sq, ww;
-        if (!__builtin_mul_overflow(sum, q[a], &sq) &&
-            !__builtin_mul_overflow(w[a], w[a], &ww)) {
+        if (mul_exact(sum, q[a], &sq) && mul_exact(w[a], w[a], &ww)) {
           delta_sq = to_double(sq - ww) / (4.0 * sum_d);
         } else {
           const long double S = static_cast<long double>(sum);

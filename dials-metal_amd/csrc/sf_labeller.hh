@@ -117,9 +117,10 @@ public:
   const Options &options() const { return options_; }
 
 private:
+  // A node's component record is comps_[i]: the passes over nodes read
+  // these small records, and only emit reads the moments.
   struct Node {
     int64_t frame;
-    sf_component c;
     uint32_t begin, end;   // pixel range in the frame's arena
     bool retained = false; // the range is in retained_ instead
     bool dead = false;     // its component was emitted or dropped
@@ -151,6 +152,7 @@ private:
   // node as its root; next_ chains a component's nodes from its root, and
   // tail_ (at the root) ends the chain.
   std::vector<Node> nodes_;
+  std::vector<sf_component> comps_;
   std::vector<uint32_t> parent_, next_, tail_;
   size_t live_nodes_ = 0;
   size_t previous_ = 0;          // where the last frame's nodes start
