@@ -75,7 +75,8 @@ double to_double(u128 v) {
   const uint64_t lo = static_cast<uint64_t>(v);
   if (hi == 0)
     return static_cast<double>(lo);
-  return std::ldexp(static_cast<double>(hi), 64) + static_cast<double>(lo);
+  // Scaling by 2^64 is exact (a power of two, far from overflow).
+  return static_cast<double>(hi) * 0x1p64 + static_cast<double>(lo);
 }
 
 } // namespace
