@@ -1,7 +1,8 @@
 """End to end on the device: EIGER2 4M frames resident in HBM through the
-threshold, the device labelling and the grouping into a DIALS reflection
-table, against the reference's own CPU path (dext, Labeller and refl::write,
-compiled from its sources into oracle/_ref) on the same frames.
+threshold, the device labelling and the grouping (pixels on the host, or
+kept on the device) into a DIALS reflection table, against the reference's
+own CPU path (dext, Labeller and refl::write, compiled from its sources into
+oracle/_ref) on the same frames.
 
 Every column of the two tables is byte-identical except the spots' variances,
 which the device reduces as exact integer moments and the reference sums in
@@ -30,15 +31,16 @@ def columns(blob):
         k: (t, n, b) for k, (t, (n, b)) in table["data"].items()}
 
 
+@pytest.mark.parametrize("device_pixels", [False, True])
 @pytest.mark.parametrize("mode", ["3d", "2d"])
-def test_frames_to_reflection_table(device, tmp_path, mode):
+def test_frames_to_reflection_table(device, tmp_path, mode, device_pixels):
     sf = device
     ref = O.reference()
     assert ref is not None, "oracle/_ref is not built"
     desc = sf.synth_desc("4m", spots_per_frame=600, persist=2, seed=91)
     h, w = desc.height, desc.width
     ws = sf.Workspace()
-    lab = sf.Labeller(h, w, two_d=(mode == "2d"))
+    lab = sf.Labeller(h, w, two_d=(mode == "2d"), device_pixels=device_pixels)
     rlab = ref.labeller(h, w, two_d=(mode == "2d"))
     for z in range(4):
         frame = sf.synth_frame(desc, z)
