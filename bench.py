@@ -315,8 +315,38 @@ def pipeline_3d(sf, dev, h, w, batches=4):
     """The whole spot-finding chain in 3D, twice: with the grouping's pixels
     on the host, and with them kept on the device (the host stitch then sees
     components only)."""
-    return {kind: pipeline_3d_run(sf, dev, h, w, batches, kind == "device_pixels")
-            for kind in ("host_pixels", "device_pixels")}
+    out = {kind: pipeline_3d_run(sf, dev, h, w, batches, kind == "device_pixels")
+           for kind in ("host_pixels", "device_pixels")}
+    out["cpu_baseline"] = grouping_baseline(sf, dev, h, w, batches,
+                                            out["device_pixels"]["spots"])
+    return out
+
+
+def grouping_baseline(sf, dev, h, w, batches, spots):
+    """The reference's own Labeller (oracle/_ref, compiled from its sources) on
+    one host thread, over the same consecutive frames' signal lists as the
+    device pipeline groups: the grouping step alone."""
+    from oracle import oracle as O
+    lib, kind = O.reference(), "reference"
+    if lib is None:
+        lib, kind = O.oracle(), "port"
+    b = sf.Batch()
+    b.threshold([(d.ptr, h, w, 32) for d in dev], graph=False)
+    counts, _ = b.sync()
+    lists = [b.fetch(i, counts[i]) for i in range(len(dev))]
+    b.close()
+    lab = lib.labeller(h, w)
+    t0 = time.perf_counter()
+    for step in range(batches):
+        for i, signal in enumerate(lists):
+            lab.add(step * len(dev) + i, signal)
+    lab.finish()
+    secs = time.perf_counter() - t0
+    n = batches * len(dev)
+    return {"value": n / secs, "unit": "frames/s", "cores": 1, "kind": kind,
+            "sample": f"the same {n} frames' signal lists through the reference "
+                      "Labeller (3D), grouping only",
+            "spots": len(lab.spots()), "same_spot_count": len(lab.spots()) == spots}
 
 
 def pipeline_3d_run(sf, dev, h, w, batches, device_pixels):
