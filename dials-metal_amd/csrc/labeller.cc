@@ -222,6 +222,15 @@ void Labeller::add_device(int64_t frame, const void *pixels, uint64_t owner,
     start[c + 1] = start[c] + comps[c].n;
   if (start[ncomp] != n)
     throw std::runtime_error("device labelling: component sizes disagree");
+  // Every check before the frame is taken: the caller releases the frames
+  // this throws for.
+  if (!options_.two_d && started_ && last_frame_ + 1 == frame &&
+      links_to_previous) {
+    const size_t prev_count = nodes_.size() - previous_;
+    for (size_t l = 0; l < nlinks; l++)
+      if (links[2 * l] >= ncomp || links[2 * l + 1] >= prev_count)
+        throw std::runtime_error("device labelling: a frame link out of range");
+  }
   begin_frame(frame);
   Arena &arena = arenas_[frame];
   arena.device = pixels;
