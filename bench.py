@@ -463,9 +463,10 @@ def main():
                          "traffic against which HBM counters are scaled")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU: this build has no multi-GPU path "
-              f"(asked for --gpus {args.gpus}); DESIGN.md describes how the "
-              f"frames would shard", file=sys.stderr)
+        print(f"bench.py measures one GPU: there is no multi-GPU path "
+              f"(asked for --gpus {args.gpus}). Frames would shard as "
+              f"contiguous frame ranges, one per GPU, with the 3D grouping "
+              f"joined across each range boundary", file=sys.stderr)
         sys.exit(2)
 
     import spotfinder_hip as sf
