@@ -237,6 +237,10 @@ void Labeller::add_device(int64_t frame, const void *pixels, uint64_t owner,
   arena.owner = owner;
   add_nodes(frame, arena, start, n, comps, ncomp, links, nlinks,
             links_to_previous);
+  // A frame without pixels is read by nothing.
+  auto it = arenas_.find(frame);
+  if (it != arenas_.end() && it->second.live == 0)
+    release(it);
 }
 void Labeller::begin_frame(int64_t frame) {
   if (started_ && frame != last_frame_ + 1)
@@ -372,7 +376,10 @@ void Labeller::flush(int64_t keep_frame) {
   // their pixels to the retained store and is released, so memory follows
   // the live pixels and not the sweep's length.  The retained store is
   // rewritten once it is mostly closed components'.
-  for (auto it = arenas_.begin(); it != arenas_.end() && it->first < keep_frame;) {
+  // (Device mode keeps each frame until its last node closes, and releases
+  // it then: nothing to do here.)
+  for (auto it = arenas_.begin();
+       !device_ && it != arenas_.end() && it->first < keep_frame;) {
     Arena &arena = it->second;
     if (arena.live > 0 &&
         (device_ || arena.live * kRetainRatio >= arena.index.size())) {
