@@ -924,6 +924,10 @@ int sf_labeller_create_device(size_t height, size_t width,
   });
 }
 
+size_t sf_labeller_device_bytes(const sf_labeller *l) {
+  return l && l->dp ? l->dp->held_bytes() : 0;
+}
+
 void sf_labeller_destroy(sf_labeller *l) { delete l; }
 
 int sf_labeller_add(sf_labeller *l, int64_t frame,

@@ -234,6 +234,7 @@ def lib():
                                 ctypes.POINTER(vp)], ctypes.c_int),
         "sf_labeller_create_device": ([sz, sz, ctypes.POINTER(Grouping),
                                        ctypes.POINTER(vp)], ctypes.c_int),
+        "sf_labeller_device_bytes": ([vp], sz),
         "sf_labeller_destroy": ([vp], None),
         "sf_labeller_add": ([vp, ctypes.c_int64, vp, sz], ctypes.c_int),
         "sf_labeller_add_ws": ([vp, ctypes.c_int64, vp], ctypes.c_int),
@@ -654,6 +655,10 @@ class Labeller:
 
     def finish(self):
         _check(lib().sf_labeller_finish(self.h))
+
+    def device_bytes(self):
+        """Device memory the staged frames still hold (device_pixels)."""
+        return int(lib().sf_labeller_device_bytes(self.h))
 
     def spots(self):
         n = lib().sf_labeller_spot_count(self.h)

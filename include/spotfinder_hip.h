@@ -311,6 +311,10 @@ int sf_labeller_create(size_t height, size_t width, const sf_grouping *g,
  * frames an open component still touches. */
 int sf_labeller_create_device(size_t height, size_t width,
                               const sf_grouping *g, sf_labeller **out);
+/* Device bytes the staged frames of a sf_labeller_create_device labeller
+ * still hold (0 once finished, and for a host labeller).  No counterpart in
+ * the reference, whose pixels are all on the host. */
+size_t sf_labeller_device_bytes(const sf_labeller *l);
 void sf_labeller_destroy(sf_labeller *l);
 int sf_labeller_add(sf_labeller *l, int64_t frame,
                     const sf_signal_pixel *signal, size_t n);
