@@ -637,8 +637,9 @@ void Labeller::emit(const uint32_t *members, size_t count) {
         nodes_[members[k]].retained ? retained_ : arenas_.at(z);
     if (e == k + 1) {
       const Node &node = nodes_[members[k]];
+      sf_pixel *dst = kept_.extend(node.end - node.begin);
       for (size_t i = node.begin; i < node.end; i++)
-        kept_.push_back(sf_pixel{static_cast<int32_t>(z), arena.index[i], arena.value[i]});
+        *dst++ = sf_pixel{static_cast<int32_t>(z), arena.index[i], arena.value[i]};
       k = e;
       continue;
     }
