@@ -1,8 +1,10 @@
 """The serving loop (sf_server_*) on the device.
 
   * alone and in the mix: the frames of three sweeps (two EIGER2 4M uint32,
-    one 512 x 512 uint16 ROI) served one at a time, then all submitted at
-    once so that the worker batches them together across sweeps; every
+    one 512 x 512 uint16 ROI), and of four of the serving configuration's
+    sizes (9M, 16M and 4M uint32, a uint16 ROI), served one at a time, then
+    all submitted at once so that the worker batches them together across
+    sweeps; every
     frame's ordered list, signal count, component count and link count are
     identical both ways, the lists equal the oracle's threshold, and the mix
     really was served in shared batches;
@@ -21,6 +23,11 @@ SWEEPS = [("4m", dict(seed=31, spots_per_frame=600, persist=2)),
           ("4m", dict(seed=32, spots_per_frame=300, persist=1)),
           ((512, 512), dict(bits=16, seed=33, spots_per_frame=40,
                             ring_radius=150.0))]
+SWEEPS_LARGE = [("9m", dict(seed=41, spots_per_frame=1200, persist=1)),
+                ("16m", dict(seed=42, spots_per_frame=2000, persist=1)),
+                ("4m", dict(seed=43, spots_per_frame=600, persist=1)),
+                ((512, 512), dict(bits=16, seed=44, spots_per_frame=40,
+                                  ring_radius=150.0))]
 FRAMES = 4
 
 
@@ -29,14 +36,23 @@ def say(*a):
     sys.stdout.flush()
 
 
-@pytest.fixture(scope="module")
-def data(device):
+def make_data(device, sweeps, frames_per_sweep):
     out = []
-    for kind, kw in SWEEPS:
+    for kind, kw in sweeps:
         d = device.synth_desc(kind, **kw)
-        frames = [device.synth_frame(d, z) for z in range(FRAMES)]
+        frames = [device.synth_frame(d, z) for z in range(frames_per_sweep)]
         out.append((d, frames, [device.DeviceArray.from_numpy(f) for f in frames]))
     return out
+
+
+@pytest.fixture(scope="module")
+def data(device):
+    return make_data(device, SWEEPS, FRAMES)
+
+
+@pytest.fixture(scope="module")
+def data_large(device):
+    return make_data(device, SWEEPS_LARGE, 3)
 
 
 def capacity(device, d):
@@ -44,10 +60,11 @@ def capacity(device, d):
 
 
 def serve(device, data, together):
-    srv = device.Server(max_batch=64, ring=FRAMES + 1, label=True)
+    frames = len(data[0][1])
+    srv = device.Server(max_batch=64, ring=frames + 1, label=True)
     ids = [srv.open_sweep(d.height, d.width, d.bits) for d, _, _ in data]
     results = {}
-    order = [(s, z) for z in range(FRAMES) for s in range(len(data))]
+    order = [(s, z) for z in range(frames) for s in range(len(data))]
     pending = []
     for s, z in order:
         d, _, dev = data[s]
@@ -67,6 +84,14 @@ def serve(device, data, together):
 
 
 def test_each_result_is_the_same_alone_and_in_the_mix(device, data, oracle_lib):
+    alone_and_mixed(device, data, oracle_lib)
+
+
+def test_large_sweeps_alone_and_in_the_mix(device, data_large, oracle_lib):
+    alone_and_mixed(device, data_large, oracle_lib)
+
+
+def alone_and_mixed(device, data, oracle_lib):
     say("server: alone")
     alone, st_alone = serve(device, data, together=False)
     say("server: in the mix", st_alone)
