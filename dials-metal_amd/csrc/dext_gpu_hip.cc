@@ -52,17 +52,18 @@ int find(const T *image_in, std::vector<SignalPixel> &signal_out,
                    : SF_KERNEL_FUSED;
   if (std::getenv("SPOTFINDER_HIP_KERNEL"))
     kernel = sf_get_kernel();
-  sf_set_kernel(kernel);
   if (kernel == SF_KERNEL_FUSED)
     internal::announce_fused();
   else
     internal::announce_windows(Window::Direct, Window::Direct);
   const bool profiling = profile_stages();
-  sf_profile_stages(profiling ? 1 : 0);
 
+  // The kernel and the timing go with this call only: other threads may be
+  // inside find() under another choice.
   std::size_t found = 0;
   const int bits = static_cast<int>(8 * sizeof(T));
-  const int rc = sf_find_run(bits, image_in, height, width, &found);
+  const int rc = sf_find_run_as(bits, image_in, height, width, kernel,
+                                profiling ? 1 : 0, &found);
   if (rc == -1 || rc == -2)
     return rc;
   if (rc != 0)

@@ -1,70 +1,126 @@
-// The backend-independent part of namespace gpu for a standalone build:
-// which windows were asked for (with the SPOTFINDER_GPU_STAGE0 / _STAGE2 /
-// _FUSED environment overrides and a warning on a value that is neither),
-// whether to profile, the per-thread stage times, and the announcement on
-// stderr whenever the configuration changes.  The behaviour is that of the
-// reference's src/spots/dext_gpu.cc:17-169, which replaces this file when the
-// backend is built into the reference (dext_gpu_hip.cc then links against
-// it); the code is this library's own.
+// The backend-independent part of namespace gpu, for a build of this library
+// on its own: the window choice, the fused switch, the profiling and
+// reporting switches, the per-thread stage times and the stderr line that
+// names the configuration when it changes.  In a build inside the reference
+// the reference's own src/spots/dext_gpu.cc takes this file's place and
+// dext_gpu_hip.cc links against that instead; both meet it through
+// dext_gpu_internal.hh.
+//
+// Interface kept from the reference (dext_gpu.hh:50-134, dext_gpu.cc:56-96,
+// 146-166): the variables SPOTFINDER_GPU_STAGE0 / _STAGE2 ("tile" or
+// "direct") and SPOTFINDER_GPU_FUSED ("1" or "0"), read once, before the first
+// read or write of a choice, so that an explicit setter call is never undone
+// by the environment; a value that is neither word is a warning and leaves the
+// default; the announcement text, printed only while reporting is on and only
+// when the configuration differs from the last one announced.
+//
+// Design: the whole configuration is one atomic word of flag bits, built once
+// by a function-local static from the backend's defaults and a table of
+// environment switches.  Getters and setters are single atomic loads and
+// read-modify-writes on that word, so a switch in one thread is seen by the
+// next find() in any thread, and nothing here takes a lock after start-up.
 #include <atomic>
 #include <cstdio>
 #include <cstdlib>
 #include <cstring>
-#include <mutex>
 
 #include "dext_gpu_internal.hh"
 
 namespace gpu {
 namespace {
 
-std::atomic<Window> chosen0{Window::Tile};
-std::atomic<Window> chosen2{Window::Tile};
-std::atomic<bool> chosen_fused{false};
-std::atomic<bool> profiling{false};
-std::atomic<bool> reporting{true};
-// The configuration last announced: -1 none, 1 fused, else 2 + 2 * stage0 +
-// stage2 (an announcement is made whenever this changes).
-std::atomic<int> announced{-1};
-std::once_flag chosen_once;
+enum Flag : unsigned {
+  StageZeroTile = 1u << 0, // stage 0 sums its window from a tile
+  StageTwoTile = 1u << 1,  // stage 2 likewise
+  Fused = 1u << 2,         // the three stages as one kernel
+  Profiling = 1u << 3,     // find() records the stage split
+  Quiet = 1u << 4,         // report_windows(false)
+};
 
-StageTimes &thread_times() {
-  static thread_local StageTimes times;
-  return times;
-}
+// One environment switch: the variable, the flag it drives and the words
+// that set and clear it.  The warning names the two words in this order.
+struct Switch {
+  const char *variable;
+  Flag flag;
+  const char *set_word;
+  const char *clear_word;
+};
 
-void resolve_one(const char *suffix, std::atomic<Window> &into) {
-  char variable[64];
-  std::snprintf(variable, sizeof variable, "SPOTFINDER_GPU_%s", suffix);
-  const char *asked = std::getenv(variable);
-  if (!asked)
-    return;
-  if (std::strcmp(asked, "direct") == 0)
-    into.store(Window::Direct);
-  else if (std::strcmp(asked, "tile") == 0)
-    into.store(Window::Tile);
-  else
-    std::fprintf(stderr,
-                 "warning: %s=%s is not 'tile' or 'direct'; leaving it alone\n",
-                 variable, asked);
-}
+constexpr Switch kSwitches[] = {
+    {"SPOTFINDER_GPU_STAGE0", StageZeroTile, "tile", "direct"},
+    {"SPOTFINDER_GPU_STAGE2", StageTwoTile, "tile", "direct"},
+    {"SPOTFINDER_GPU_FUSED", Fused, "1", "0"},
+};
 
-void resolve() {
-  chosen0.store(internal::default_stage0_window());
-  chosen2.store(internal::default_stage2_window());
-  resolve_one("STAGE0", chosen0);
-  resolve_one("STAGE2", chosen2);
-  if (const char *asked = std::getenv("SPOTFINDER_GPU_FUSED")) {
-    if (std::strcmp(asked, "1") == 0)
-      chosen_fused.store(true);
-    else if (std::strcmp(asked, "0") == 0)
-      chosen_fused.store(false);
+unsigned apply_environment(unsigned word) {
+  for (const Switch &s : kSwitches) {
+    const char *value = std::getenv(s.variable);
+    if (value == nullptr)
+      continue;
+    if (std::strcmp(value, s.set_word) == 0)
+      word |= s.flag;
+    else if (std::strcmp(value, s.clear_word) == 0)
+      word &= ~static_cast<unsigned>(s.flag);
     else
       std::fprintf(stderr,
-                   "warning: SPOTFINDER_GPU_FUSED=%s is not '1' or '0'; "
-                   "leaving it alone\n",
-                   asked);
+                   "warning: %s=%s is not '%s' or '%s'; leaving it alone\n",
+                   s.variable, value, s.set_word, s.clear_word);
   }
+  return word;
 }
+
+// Configurations as the announcement sees them: the window pair in the low
+// two bits, or the fused kernel; Shown marks a value as announced, so that
+// "nothing yet" (zero) differs from every configuration.
+constexpr unsigned kShown = 1u << 8;
+constexpr unsigned kFusedConfig = 1u << 2;
+
+class Policy {
+public:
+  static Policy &instance() {
+    static Policy policy; // initialised once, thread-safely, on first use
+    return policy;
+  }
+
+  bool has(Flag f) const {
+    return (word_.load(std::memory_order_relaxed) & f) != 0;
+  }
+  void put(Flag f, bool on) {
+    if (on)
+      word_.fetch_or(f, std::memory_order_relaxed);
+    else
+      word_.fetch_and(~static_cast<unsigned>(f), std::memory_order_relaxed);
+  }
+
+  // True when `config` should be announced now: reporting is on and it is
+  // not the configuration announced last.  Two threads switching at the same
+  // moment may both announce, which repeats a line but never misstates one.
+  bool should_announce(unsigned config) {
+    if (has(Quiet))
+      return false;
+    const unsigned mark = config | kShown;
+    return last_shown_.exchange(mark, std::memory_order_relaxed) != mark;
+  }
+
+private:
+  Policy() {
+    unsigned word = 0;
+    if (internal::default_stage0_window() == Window::Tile)
+      word |= StageZeroTile;
+    if (internal::default_stage2_window() == Window::Tile)
+      word |= StageTwoTile;
+    word_.store(apply_environment(word), std::memory_order_relaxed);
+  }
+
+  std::atomic<unsigned> word_{0};
+  std::atomic<unsigned> last_shown_{0};
+};
+
+Window window_of(Flag f) {
+  return Policy::instance().has(f) ? Window::Tile : Window::Direct;
+}
+
+thread_local StageTimes t_last_split;
 
 } // namespace
 
@@ -72,54 +128,37 @@ const char *name(Window window) {
   return window == Window::Tile ? "tile" : "direct";
 }
 
-Window stage0_window() {
-  std::call_once(chosen_once, resolve);
-  return chosen0.load(std::memory_order_relaxed);
+Window stage0_window() { return window_of(StageZeroTile); }
+Window stage2_window() { return window_of(StageTwoTile); }
+void stage0_window(Window w) {
+  Policy::instance().put(StageZeroTile, w == Window::Tile);
 }
-Window stage2_window() {
-  std::call_once(chosen_once, resolve);
-  return chosen2.load(std::memory_order_relaxed);
+void stage2_window(Window w) {
+  Policy::instance().put(StageTwoTile, w == Window::Tile);
 }
-void stage0_window(Window window) {
-  std::call_once(chosen_once, resolve);
-  chosen0.store(window, std::memory_order_relaxed);
-}
-void stage2_window(Window window) {
-  std::call_once(chosen_once, resolve);
-  chosen2.store(window, std::memory_order_relaxed);
-}
-bool fused() {
-  std::call_once(chosen_once, resolve);
-  return chosen_fused.load(std::memory_order_relaxed);
-}
-void fused(bool on) {
-  std::call_once(chosen_once, resolve);
-  chosen_fused.store(on, std::memory_order_relaxed);
-}
+bool fused() { return Policy::instance().has(Fused); }
+void fused(bool on) { Policy::instance().put(Fused, on); }
 
-void profile_stages(bool on) { profiling.store(on, std::memory_order_relaxed); }
-bool profile_stages() { return profiling.load(std::memory_order_relaxed); }
-StageTimes last_stage_times() { return thread_times(); }
-void report_windows(bool on) { reporting.store(on, std::memory_order_relaxed); }
+void profile_stages(bool on) { Policy::instance().put(Profiling, on); }
+bool profile_stages() { return Policy::instance().has(Profiling); }
+StageTimes last_stage_times() { return t_last_split; }
+void report_windows(bool on) { Policy::instance().put(Quiet, !on); }
 
 namespace internal {
 
-void set_stage_times(const StageTimes &times) { thread_times() = times; }
+void set_stage_times(const StageTimes &times) { t_last_split = times; }
 
 void announce_windows(Window stage0, Window stage2) {
-  const int key = 2 + 2 * static_cast<int>(stage0) + static_cast<int>(stage2);
-  if (announced.exchange(key, std::memory_order_relaxed) == key ||
-      !reporting.load(std::memory_order_relaxed))
-    return;
-  std::fprintf(stderr, "%s: stage0 %s window, stage2 %s window\n", backend(),
-               name(stage0), name(stage2));
+  const unsigned config = (stage0 == Window::Tile ? 2u : 0u) |
+                          (stage2 == Window::Tile ? 1u : 0u);
+  if (Policy::instance().should_announce(config))
+    std::fprintf(stderr, "%s: stage0 %s window, stage2 %s window\n", backend(),
+                 name(stage0), name(stage2));
 }
 
 void announce_fused() {
-  if (announced.exchange(1, std::memory_order_relaxed) == 1 ||
-      !reporting.load(std::memory_order_relaxed))
-    return;
-  std::fprintf(stderr, "%s: the three stages fused, one kernel\n", backend());
+  if (Policy::instance().should_announce(kFusedConfig))
+    std::fprintf(stderr, "%s: the three stages fused, one kernel\n", backend());
 }
 
 } // namespace internal

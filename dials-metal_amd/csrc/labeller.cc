@@ -160,23 +160,27 @@ void Labeller::merge(uint32_t a, uint32_t b) {
   tail_[keep] = tail_[join];
 }
 
+void Labeller::check_links(int64_t frame, size_t ncomp, const uint32_t *links,
+                           size_t nlinks, bool links_to_previous) const {
+  if (options_.two_d || !started_ || last_frame_ + 1 != frame ||
+      !links_to_previous)
+    return;
+  const size_t prev_count = nodes_.size() - previous_;
+  for (size_t l = 0; l < nlinks; l++)
+    if (links[2 * l] >= ncomp || links[2 * l + 1] >= prev_count)
+      throw std::runtime_error("device labelling: a frame link out of range");
+}
+
 void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
                    const uint32_t *comp_of, const sf_component *comps,
                    size_t ncomp, const uint32_t *links, size_t nlinks,
                    bool links_to_previous) {
   if (device_)
     throw std::runtime_error("dials_spots: this labeller takes device frames");
+  // Every check before any state changes, so that a refused frame leaves
+  // the spots, pixels and counts as they were and the next frame groups as
+  // if it had never been offered.
   check_frame(frame, signal, n);
-  begin_frame(frame);
-
-  // Pixels of this frame grouped by component, each group still ascending.
-  Arena &arena = arenas_[frame];
-  if (!spare_arenas_.empty()) { // a released frame's buffers, capacity kept
-    arena = std::move(spare_arenas_.back());
-    spare_arenas_.pop_back();
-  }
-  arena.index.resize(n);
-  arena.value.resize(n);
   std::vector<uint32_t> &start = start_;
   start.assign(ncomp + 1, 0);
   uint32_t top = 0;
@@ -186,6 +190,20 @@ void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
     throw std::runtime_error("device labelling: component out of range");
   for (size_t i = 0; i < n; i++)
     start[comp_of[i] + 1]++;
+  for (size_t c = 0; c < ncomp; c++)
+    if (start[c + 1] != comps[c].n)
+      throw std::runtime_error("device labelling: component sizes disagree");
+  check_links(frame, ncomp, links, nlinks, links_to_previous);
+
+  begin_frame(frame);
+  // Pixels of this frame grouped by component, each group still ascending.
+  Arena &arena = arenas_[frame];
+  if (!spare_arenas_.empty()) { // a released frame's buffers, capacity kept
+    arena = std::move(spare_arenas_.back());
+    spare_arenas_.pop_back();
+  }
+  arena.index.resize(n);
+  arena.value.resize(n);
   for (size_t c = 0; c < ncomp; c++)
     start[c + 1] += start[c];
   std::vector<uint32_t> &at = at_;
@@ -224,13 +242,7 @@ void Labeller::add_device(int64_t frame, const void *pixels, uint64_t owner,
     throw std::runtime_error("device labelling: component sizes disagree");
   // Every check before the frame is taken: the caller releases the frames
   // this throws for.
-  if (!options_.two_d && started_ && last_frame_ + 1 == frame &&
-      links_to_previous) {
-    const size_t prev_count = nodes_.size() - previous_;
-    for (size_t l = 0; l < nlinks; l++)
-      if (links[2 * l] >= ncomp || links[2 * l + 1] >= prev_count)
-        throw std::runtime_error("device labelling: a frame link out of range");
-  }
+  check_links(frame, ncomp, links, nlinks, links_to_previous);
   begin_frame(frame);
   Arena &arena = arenas_[frame];
   arena.device = pixels;
@@ -258,8 +270,6 @@ void Labeller::add_nodes(int64_t frame, Arena &arena,
     node.frame = frame;
     node.begin = start[c];
     node.end = start[c + 1];
-    if (node.end - node.begin != comps[c].n)
-      throw std::runtime_error("device labelling: component sizes disagree");
     const uint32_t self = static_cast<uint32_t>(nodes_.size());
     nodes_.push_back(node);
     comps_.push_back(comps[c]);
@@ -274,13 +284,11 @@ void Labeller::add_nodes(int64_t frame, Arena &arena,
   counts_.signal_pixels += n;
   live_pixels_ += n;
 
-  // Six-connected: components sharing a position with the previous frame.
+  // Six-connected: components sharing a position with the previous frame
+  // (the links were checked against both frames before anything changed).
   if (!options_.two_d && started_ && last_frame_ + 1 == frame && links_to_previous) {
-    const size_t prev_count = base - previous;
     for (size_t l = 0; l < nlinks; l++) {
       const uint32_t here = links[2 * l], there = links[2 * l + 1];
-      if (here >= ncomp || there >= prev_count)
-        throw std::runtime_error("device labelling: a frame link out of range");
       merge(static_cast<uint32_t>(base + here),
             static_cast<uint32_t>(previous + there));
     }

@@ -441,22 +441,39 @@ void sf_profile_stages(int on) { g_profile.store(on != 0); }
 int sf_profiling_stages(void) { return g_profile.load() ? 1 : 0; }
 sf_stage_times sf_last_stage_times(void) { return t_times; }
 
-int sf_find_run(int pixel_bits, const void *frame, size_t height, size_t width,
-                size_t *found) {
+namespace {
+// Whether this thread's last find asked for stage times (its fetch then
+// times the copy back).
+thread_local bool t_find_timing = false;
+// sf_inject_find_failure: calls left before the one that fails (0: none).
+std::atomic<uint64_t> g_fail_in{0};
+
+bool injected_failure_now() {
+  uint64_t left = g_fail_in.load();
+  while (left != 0)
+    if (g_fail_in.compare_exchange_weak(left, left - 1))
+      return left == 1;
+  return false;
+}
+
+int find_run(int pixel_bits, const void *frame, size_t height, size_t width,
+             int kernel, bool timing, size_t *found) {
   if (found)
     *found = 0;
   if (frame == nullptr || bad_shape(height, width) ||
       (pixel_bits != 16 && pixel_bits != 32))
     return bad("null frame, zero size, more than INT32_MAX pixels, or not "
                "16 or 32 bits");
+  if (kernel != SF_KERNEL_FUSED && kernel != SF_KERNEL_DIRECT)
+    return bad("kernel is neither SF_KERNEL_FUSED nor SF_KERNEL_DIRECT");
   return guarded([&]() -> int {
     auto &ws = thread_workspace();
     if (!ws)
       ws = std::make_unique<sf_workspace>(nullptr);
     const StreamGuard idle{ws->engine.stream()};
+    t_find_timing = timing;
     const size_t bytes = height * width * static_cast<size_t>(pixel_bits / 8);
     ws->upload.ensure(bytes, ws->engine.stream());
-    const bool timing = g_profile.load();
     hipEvent_t up[2] = {nullptr, nullptr};
     if (timing) {
       for (hipEvent_t &e : up)
@@ -471,7 +488,12 @@ int sf_find_run(int pixel_bits, const void *frame, size_t height, size_t width,
     ws->engine.set_timing(timing);
     ws_threshold(ws.get(), pixel_bits, ws->upload.ptr,
                  static_cast<uint32_t>(height), static_cast<uint32_t>(width),
-                 sf_get_kernel());
+                 kernel);
+    // A failure forced from the host, with the frame's work still in
+    // flight: the guard above must leave the stream idle, as after a real
+    // device failure.
+    if (injected_failure_now())
+      throw device_error("injected failure (sf_inject_find_failure)");
     const int rc = ws_sync(ws.get(), found);
     if (timing) {
       sf_stage_times t{};
@@ -486,6 +508,27 @@ int sf_find_run(int pixel_bits, const void *frame, size_t height, size_t width,
     return rc;
   });
 }
+} // namespace
+
+int sf_find_run(int pixel_bits, const void *frame, size_t height, size_t width,
+                size_t *found) {
+  return find_run(pixel_bits, frame, height, width, sf_get_kernel(),
+                  g_profile.load(), found);
+}
+
+int sf_find_run_as(int pixel_bits, const void *frame, size_t height,
+                   size_t width, int kernel, int timing, size_t *found) {
+  return find_run(pixel_bits, frame, height, width, kernel, timing != 0, found);
+}
+
+void sf_inject_find_failure(uint64_t nth) { g_fail_in.store(nth); }
+
+int sf_find_idle(void) {
+  auto &ws = thread_workspace();
+  if (!ws)
+    return 1;
+  return hipStreamQuery(ws->engine.stream()) == hipSuccess ? 1 : 0;
+}
 
 int sf_find_fetch(sf_signal_pixel *out, size_t n) {
   auto &ws = thread_workspace();
@@ -496,7 +539,7 @@ int sf_find_fetch(sf_signal_pixel *out, size_t n) {
   return guarded([&]() -> int {
     const auto t0 = std::chrono::steady_clock::now();
     ws->engine.fetch(ws->cur, out, n);
-    if (g_profile.load())
+    if (t_find_timing)
       t_times.copy = std::chrono::duration<double, std::milli>(
                          std::chrono::steady_clock::now() - t0)
                          .count();

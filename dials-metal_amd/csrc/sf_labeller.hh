@@ -140,6 +140,10 @@ private:
   static void move_pixels(const Arena &from, Node &node, Arena &to);
   void release(std::map<int64_t, Arena>::iterator it);
   void begin_frame(int64_t frame);
+  // Throws for a frame link that names a component outside this frame or
+  // the previous one; changes nothing.
+  void check_links(int64_t frame, size_t ncomp, const uint32_t *links,
+                   size_t nlinks, bool links_to_previous) const;
   void add_nodes(int64_t frame, Arena &arena, const std::vector<uint32_t> &start,
                  size_t n, const sf_component *comps, size_t ncomp,
                  const uint32_t *links, size_t nlinks, bool links_to_previous);

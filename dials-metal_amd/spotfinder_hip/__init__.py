@@ -194,6 +194,10 @@ def lib():
         "sf_last_stage_times": ([], StageTimes),
         "sf_find_run": ([ctypes.c_int, vp, sz, sz, ctypes.POINTER(sz)], ctypes.c_int),
         "sf_find_fetch": ([vp, sz], ctypes.c_int),
+        "sf_find_run_as": ([ctypes.c_int, vp, sz, sz, ctypes.c_int, ctypes.c_int,
+                            ctypes.POINTER(sz)], ctypes.c_int),
+        "sf_inject_find_failure": ([ctypes.c_uint64], None),
+        "sf_find_idle": ([], ctypes.c_int),
         "sf_ws_create": ([sz, vp, ctypes.POINTER(vp)], ctypes.c_int),
         "sf_ws_destroy": ([vp], None),
         "sf_ws_stream": ([vp], vp),
@@ -392,6 +396,17 @@ def set_tuning(**kw):
         raise ValueError(last_error())
 
 
+def inject_find_failure(nth):
+    """The nth find from now (over all threads) fails as a device failure
+    would, after its work is enqueued; 0 cancels."""
+    lib().sf_inject_find_failure(int(nth))
+
+
+def find_idle():
+    """True when this thread's find stream has no work in flight."""
+    return bool(lib().sf_find_idle())
+
+
 def set_kernel(name):
     lib().sf_set_kernel(KERNELS[name])
 
@@ -410,10 +425,11 @@ def last_stage_times():
     return {f: getattr(t, f) for f, _ in StageTimes._fields_}
 
 
-def find(frame):
+def find(frame, kernel=None):
     """gpu::find<T>: (status, SignalPixel array ascending by index) for a
     host uint16 or uint32 frame.  Status as dext(): 0, -1 or -2; a device
-    failure raises SpotfinderError."""
+    failure raises SpotfinderError.  `kernel` ("fused" or "direct") chooses
+    for this call alone; None follows set_kernel."""
     frame = np.ascontiguousarray(frame)
     if frame.dtype == np.uint16:
         bits = 16
@@ -425,8 +441,12 @@ def find(frame):
         raise ValueError("a frame is two-dimensional")
     h, w = frame.shape
     found = ctypes.c_size_t(0)
-    rc = lib().sf_find_run(bits, _ptr(frame) if frame.size else None, h, w,
-                           ctypes.byref(found))
+    if kernel is None:
+        rc = lib().sf_find_run(bits, _ptr(frame) if frame.size else None, h, w,
+                               ctypes.byref(found))
+    else:
+        rc = lib().sf_find_run_as(bits, _ptr(frame) if frame.size else None, h, w,
+                                  KERNELS[kernel], 0, ctypes.byref(found))
     if rc == -3 or rc == -4:
         _check(rc)
     if rc != 0:

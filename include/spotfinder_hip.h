@@ -163,6 +163,23 @@ int sf_find_run(int pixel_bits, const void *frame, size_t height, size_t width,
                 size_t *found);
 int sf_find_fetch(sf_signal_pixel *out, size_t n);
 
+/* sf_find_run with the kernel (SF_KERNEL_FUSED or SF_KERNEL_DIRECT) and the
+ * stage timing chosen for this call alone, not read from the process-wide
+ * settings: what gpu::find<T> calls, so that threads finding frames at once
+ * under different window choices (find_spots.cc:732-835) never steer each
+ * other's launches.  -1 also for an unknown kernel. */
+int sf_find_run_as(int pixel_bits, const void *frame, size_t height,
+                   size_t width, int kernel, int timing, size_t *found);
+
+/* Failure injection for tests of the error path (dext_cuda.cu:46-51,
+ * 445-467; find_spots.cc:410-416): the nth sf_find_run / sf_find_run_as call
+ * from now, counted over all threads, fails with -3 after its frame's work is
+ * enqueued, as a device failure would; 0 cancels.  No counterpart in the
+ * reference.  sf_find_idle is 1 when the calling thread's find stream has no
+ * work in flight (it must be, however find returns), else 0. */
+void sf_inject_find_failure(uint64_t nth);
+int sf_find_idle(void);
+
 /* ------------------------------------------------------------------------
  * Device-resident frames.  New surface, for callers that keep frames in HBM
  * (the batched and serving paths, and the benchmark).

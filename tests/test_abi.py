@@ -119,3 +119,34 @@ def test_backend_builds_against_the_reference_headers():
                         "-I" + os.path.join(ROOT, "include"), src],
                        capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_policy_environment_setters_and_announcements():
+    """gpu::'s backend-independent policy (tests/cpp/test_policy.cc), which
+    needs no device: the SPOTFINDER_GPU_* words, the warning for any other
+    value, setters over the environment, and an announcement only when the
+    configuration changes while reporting is on (dext_gpu.hh:50-134)."""
+    import subprocess
+    exe = os.path.join(ROOT, "tests", "cpp", "test_policy")
+    cases = [({}, ("tile", "tile", "0"), []),
+             ({"SPOTFINDER_GPU_STAGE0": "direct", "SPOTFINDER_GPU_FUSED": "1"},
+              ("direct", "tile", "1"), []),
+             ({"SPOTFINDER_GPU_STAGE2": "direct", "SPOTFINDER_GPU_FUSED": "0"},
+              ("tile", "direct", "0"), []),
+             ({"SPOTFINDER_GPU_STAGE0": "Tile", "SPOTFINDER_GPU_FUSED": "yes"},
+              ("tile", "tile", "0"),
+              ["warning: SPOTFINDER_GPU_STAGE0=Tile is not 'tile' or 'direct'; "
+               "leaving it alone",
+               "warning: SPOTFINDER_GPU_FUSED=yes is not '1' or '0'; leaving it alone"])]
+    for env, expect, warnings in cases:
+        full = {k: v for k, v in os.environ.items() if not k.startswith("SPOTFINDER_GPU_")}
+        full.update(env)
+        r = subprocess.run([exe, *expect], env=full, capture_output=True, text=True)
+        assert r.returncode == 0, (env, r.stdout)
+        err = r.stderr.splitlines()
+        start = err.index("<<announce")
+        assert err[:start] == warnings, env
+        assert err[start + 1:err.index("announce>>")] == [
+            "HIP: the three stages fused, one kernel",
+            "HIP: stage0 direct window, stage2 direct window",
+            "HIP: the three stages fused, one kernel"], env

@@ -227,3 +227,54 @@ def test_cpp_dropin_interface(device):
     r = subprocess.run([exe], capture_output=True, text=True, timeout=600)
     print(r.stdout[-2000:], r.stderr[-2000:])
     assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_cpp_find_from_threads_like_mxi_find(device):
+    """gpu::find from eight worker threads at once, each on its own stream,
+    in mxi_find's chunks of 4N+4 frames grouped in frame order, half of them
+    through the C ABI with the direct kernels chosen per call; then a failure
+    forced after a frame's work is enqueued throws, leaves nothing in flight,
+    and the next frame is right (tests/cpp/test_find_threads.cc)."""
+    import os
+    import subprocess
+    from conftest import ROOT
+    exe = os.path.join(ROOT, "tests", "cpp", "test_find_threads")
+    assert os.path.exists(exe), "build first: make -C dials-metal_amd tests"
+    say("test_find_threads 8")
+    r = subprocess.run([exe, "8"], capture_output=True, text=True, timeout=300)
+    print(r.stdout[-3000:], r.stderr[-2000:])
+    assert r.returncode == 0, r.stdout + r.stderr
+
+
+def test_find_kernel_per_call_from_python_threads(device, oracle_lib):
+    """sf_find_run_as: threads asking for different kernels at once each get
+    the oracle's list, and the process-wide choice is left as it was."""
+    import threading
+    rng = np.random.default_rng(5)
+    frames = []
+    for i in range(12):
+        f = rng.poisson(2.0, (300 + 7 * i, 400)).astype(np.uint32)
+        s = rng.uniform(size=f.shape) < 0.02
+        f[s] += rng.integers(50, 3000, s.sum()).astype(np.uint32)
+        frames.append(f)
+    wants = [oracle_lib.dext(f)[1] for f in frames]
+    before = device.get_kernel()
+    errors = []
+
+    def work(t):
+        try:
+            for rep in range(3):
+                for i in range(t, len(frames), 4):
+                    rc, got = device.find(frames[i], kernel="direct" if t % 2 else "fused")
+                    if rc != 0 or got.tobytes() != wants[i].tobytes():
+                        errors.append((t, rep, i))
+        except Exception as e:  # noqa: BLE001 - reported below
+            errors.append((t, repr(e)))
+
+    ts = [threading.Thread(target=work, args=(t,)) for t in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join()
+    assert not errors, errors
+    assert device.get_kernel() == before

@@ -142,3 +142,44 @@ def test_host_grouping_long_lived_components(sf):
     compare_spots(got, want.spots())
     assert ours.pixels().tobytes() == want.pixels().tobytes()
     assert ours.counts() == want.counts()
+
+
+@pytest.mark.parametrize("fault", ["link", "sizes", "component", "order"])
+def test_refused_frame_changes_nothing(sf, sweep, fault):
+    """A frame the host grouping refuses (sf_labeller_add_labelled) leaves
+    the spots, pixels and counts as they were, and the same frame offered
+    again correctly, then the rest of the sweep, groups exactly as the
+    reference's Labeller does with no refusal at all."""
+    from labelling_model import label_frame, links_between
+    lab = sf.Labeller(384, 320)
+    prev = None
+    for z in range(10):
+        sig = np.ascontiguousarray(sweep[f"signal_{z}"], sf.SIGNAL_PIXEL)
+        comp_of, comps, rank = label_frame(sig, 320)
+        links = np.zeros((0, 2), np.uint32)
+        if prev is not None:
+            links = links_between(sig, comp_of, prev[0], prev[1])
+        if z == 4:
+            before = (lab.spots().tobytes(), lab.pixels().tobytes(), lab.counts(),
+                      lab.live())
+            bad_comp_of, bad_comps, bad_links = comp_of.copy(), comps.copy(), links.copy()
+            frame = z
+            if fault == "link":
+                bad_links = np.vstack([links, [[0, 100000]]]).astype(np.uint32)
+            elif fault == "sizes":
+                bad_comps["n"][0] += 1
+            elif fault == "component":
+                bad_comp_of[-1] = len(comps)
+            else:
+                frame = 2  # before the last frame added
+            with pytest.raises(sf.LabellerError):
+                lab.add_labelled(frame, sig, bad_comp_of, bad_comps, bad_links, True)
+            assert (lab.spots().tobytes(), lab.pixels().tobytes(), lab.counts(),
+                    lab.live()) == before
+        lab.add_labelled(z, sig, comp_of, comps, links, prev is not None)
+        prev = (rank, comp_of)
+    lab.finish()
+    compare_spots(lab.spots(), sweep["spots_3d"])
+    assert lab.pixels().tobytes() == sweep["pixels_3d"].tobytes()
+    c = lab.counts()
+    assert [c[k] for k in O.COUNTS_FIELDS] == list(sweep["counts_3d"])
