@@ -41,12 +41,17 @@ Engine::Engine(void *borrowed_stream) {
     tf[m] = m >= 2 ? std::sqrt(2.0f * (static_cast<float>(m) - 1.0f)) : HUGE_VALF;
     td[m] = m >= 2 ? std::sqrt(2.0 * (static_cast<double>(m) - 1.0)) : HUGE_VAL;
   }
+  // Uploaded on the engine's own stream: a copy on the legacy stream would
+  // be an implicit dependency on any stream another thread is capturing,
+  // which invalidates that capture.
   table_.ensure(sizeof(tf) + sizeof(td), stream_);
-  check(hipMemcpy(table_.ptr, tf, sizeof(tf), hipMemcpyHostToDevice),
+  check(hipMemcpyAsync(table_.ptr, tf, sizeof(tf), hipMemcpyHostToDevice,
+                       stream_),
         "uploading the root table");
-  check(hipMemcpy(table_.as<char>() + sizeof(tf), td, sizeof(td),
-                  hipMemcpyHostToDevice),
+  check(hipMemcpyAsync(table_.as<char>() + sizeof(tf), td, sizeof(td),
+                       hipMemcpyHostToDevice, stream_),
         "uploading the root table");
+  check(hipStreamSynchronize(stream_), "uploading the root table");
   desc_.ensure(2 * kDescHalf, stream_);
   queue_.ensure(256, stream_);
 }

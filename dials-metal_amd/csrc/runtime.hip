@@ -190,12 +190,13 @@ std::unique_ptr<sf_workspace> &thread_workspace() {
 }
 
 // The workspace's next slot takes the frame; it links to the one before.
+// With `graph` the fused sequence replays from one capture per frame shape.
 void ws_threshold(sf_workspace *ws, int bits, const void *frame, uint32_t h,
-                  uint32_t w, int kernel) {
+                  uint32_t w, int kernel, bool graph = true) {
   ws->cur ^= 1;
   const FrameIn f{frame, h, w, bits};
-  // The fused sequence replays from one capture per frame shape.
-  ws->engine.threshold(&f, &ws->cur, 1, kernel, kernel == SF_KERNEL_FUSED);
+  ws->engine.threshold(&f, &ws->cur, 1, kernel,
+                       graph && kernel == SF_KERNEL_FUSED);
 }
 
 int ws_sync(sf_workspace *ws, size_t *found) {
@@ -486,9 +487,12 @@ int find_run(int pixel_bits, const void *frame, size_t height, size_t width,
     if (timing)
       check(hipEventRecord(up[1], ws->engine.stream()), "hipEventRecord");
     ws->engine.set_timing(timing);
+    // No graph capture here: find runs on many threads at once, and another
+    // thread's legacy-stream work during a capture would invalidate it.  A
+    // host-fed frame's upload outweighs the launches a replay would save.
     ws_threshold(ws.get(), pixel_bits, ws->upload.ptr,
                  static_cast<uint32_t>(height), static_cast<uint32_t>(width),
-                 kernel);
+                 kernel, false);
     // A failure forced from the host, with the frame's work still in
     // flight: the guard above must leave the stream idle, as after a real
     // device failure.
