@@ -198,6 +198,10 @@ def lib():
                             ctypes.POINTER(sz)], ctypes.c_int),
         "sf_inject_find_failure": ([ctypes.c_uint64], None),
         "sf_find_idle": ([], ctypes.c_int),
+        "sf_threshold_noalloc": ([ctypes.c_int, vp, u32, u32, vp, sz,
+                                  ctypes.POINTER(sz), vp], ctypes.c_int),
+        "sf_noalloc_scratch_bytes": ([], sz),
+        "sf_noalloc_output_records": ([u32, u32, ctypes.c_int], sz),
         "sf_ws_create": ([sz, vp, ctypes.POINTER(vp)], ctypes.c_int),
         "sf_ws_destroy": ([vp], None),
         "sf_ws_stream": ([vp], vp),
@@ -405,6 +409,25 @@ def inject_find_failure(nth):
 def find_idle():
     """True when this thread's find stream has no work in flight."""
     return bool(lib().sf_find_idle())
+
+
+def threshold_noalloc(dev_frame, bits, height, width, dev_out, out_records,
+                      stream=None):
+    """sf_threshold_noalloc: thresholds a device frame into the device list
+    `dev_out` (out_records entries) without allocating; (status, count).
+    -1 raises ValueError with the library's message; -3 raises."""
+    found = ctypes.c_size_t(0)
+    rc = lib().sf_threshold_noalloc(bits, dev_frame, height, width, dev_out,
+                                    out_records, ctypes.byref(found), stream)
+    if rc == -1:
+        raise ValueError(last_error())
+    if rc not in (0, -2):
+        _check(rc)
+    return rc, found.value
+
+
+def noalloc_output_records(height, width, bits):
+    return int(lib().sf_noalloc_output_records(height, width, bits))
 
 
 def set_kernel(name):

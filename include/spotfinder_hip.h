@@ -181,6 +181,32 @@ void sf_inject_find_failure(uint64_t nth);
 int sf_find_idle(void);
 
 /* ------------------------------------------------------------------------
+ * Allocation-free threshold.  No counterpart in the reference: for callers
+ * that cannot allocate device memory (code under stream capture, tight
+ * serving loops, harnesses that forbid hipMalloc).
+ *
+ * sf_threshold_noalloc is sf_ws_threshold + sf_ws_sync for one device frame
+ * (pixel_bits 16 or 32), with the ordered SignalPixel list written to the
+ * caller's device list `dev_out` (out_records entries, at least
+ * sf_noalloc_output_records(height, width, pixel_bits): the frame's list
+ * capacity, frame bytes / 16, and never under one record per 16 pixels) and
+ * the count to *found.  It calls no allocator: every other buffer is in a
+ * static device array of sf_noalloc_scratch_bytes() (80 MiB), which each
+ * device holds until the process ends.  Frames whose scratch does not fit
+ * in it are refused with -1 and a message (every frame up to 4148 x 4362 at
+ * 32 bits fits).  Work is enqueued on `stream` (NULL: the null stream) and
+ * waited for.  0; -1 bad argument; -2 more signal pixels than the list's
+ * capacity (*found is the count, the list is not written); -3 device
+ * failure.  All calls on one device share its scratch, which keeps the last
+ * call's contents: calls that overlap in time on one device race.
+ * ------------------------------------------------------------------------ */
+int sf_threshold_noalloc(int pixel_bits, const void *dev_frame, uint32_t height,
+                         uint32_t width, sf_signal_pixel *dev_out,
+                         size_t out_records, size_t *found, void *stream);
+size_t sf_noalloc_scratch_bytes(void);
+size_t sf_noalloc_output_records(uint32_t height, uint32_t width, int pixel_bits);
+
+/* ------------------------------------------------------------------------
  * Device-resident frames.  New surface, for callers that keep frames in HBM
  * (the batched and serving paths, and the benchmark).
  *
