@@ -24,6 +24,7 @@ decoded into frames on the device.  The `narrow` object measures
 the `refl` object the reflection-table writer beside the reference's.
 """
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -62,21 +63,95 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def cpu_baseline(frames_host, threads):
+def host_cores():
+    """The host cores this process can run on, as mxi_find's default -j
+    (std::thread::hardware_concurrency(), find_spots.cc:276-278) would use
+    them: the CPUs in this process's affinity mask, bounded by a cgroup CPU
+    quota where one is set; with what was read, for the record."""
+    info = {"cpu_count": os.cpu_count()}
+    n = len(os.sched_getaffinity(0))
+    info["affinity"] = n
+    try:
+        with open("/sys/fs/cgroup/cpu.max") as fh:
+            quota, period = fh.read().split()
+        if quota != "max":
+            q = -(-int(quota) // int(period))
+            info["cgroup_quota_cpus"] = q
+            n = min(n, max(1, q))
+    except (OSError, ValueError):
+        pass
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    info["model"] = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return n, info
+
+
+def cpu_baseline(frames_host):
     """The reference's dext<uint32_t> (oracle/_ref, compiled from the
-    reference's own sources) on this host's cores, a bounded sample."""
+    reference's own sources) on every host core this process may use, one
+    frame per thread at a time as mxi_find -j N spreads them, over a bounded
+    sample: two frames a thread, cycled from the benchmark's frames."""
     from oracle import oracle as O
+    threads, info = host_cores()
     lib, kind = O.reference(), "reference"
     if lib is None:
         lib, kind = O.oracle(), "port"
-    n = min(len(frames_host), 2 * threads)
-    sample = np.stack(frames_host[:n])
-    signal, secs = lib.threads_u32(sample, threads)
-    return {"value": n / secs, "unit": "frames/s", "cores": threads, "kind": kind,
-            "sample": f"{n} of the benchmark's 16M uint32 frames through "
-                      f"dext<uint32_t> on {threads} threads, one frame per "
-                      f"thread at a time: {secs:.2f} s wall, {signal} signal "
-                      f"pixels; threshold only, no grouping"}
+    total = 2 * threads
+    sample = np.stack(frames_host)
+    signal, secs = lib.threads_u32(sample, threads, total=total)
+    return {"value": total / secs, "unit": "frames/s", "cores": threads, "kind": kind,
+            "host": info,
+            "sample": f"{total} of the benchmark's 16M uint32 frames (its "
+                      f"{len(frames_host)} cycled) through dext<uint32_t> on "
+                      f"{threads} threads, one frame per thread at a time: "
+                      f"{secs:.2f} s wall, {signal} signal pixels; threshold "
+                      f"only, no grouping"}
+
+
+def plain_host_fed(sf, frames_host, reps=2):
+    """gpu::find's own path (dext_cuda.cu:762-849 in the reference): a host
+    frame copied to the device, thresholded and ordered there, and its list
+    copied back, one frame at a time on one thread; frames in pinned memory
+    (the reference's FrameBuffer takes host_alloc's, find_spots.cc:351-367)
+    and in ordinary pageable memory."""
+    out = {}
+    n = len(frames_host)
+    pinned = []
+    for f in frames_host[: min(n, 8)]:
+        p = sf.lib().sf_host_alloc(f.nbytes)
+        if not p:
+            raise RuntimeError("sf_host_alloc failed")
+        arr = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint32)),
+                                    shape=f.shape)
+        arr[...] = f
+        pinned.append((p, arr))
+    try:
+        for name, frames in (("pinned", [a for _, a in pinned]),
+                             ("pageable", frames_host[: len(pinned)])):
+            for f in frames[:2]:
+                sf.find(f)
+            t0 = time.perf_counter()
+            count = 0
+            for _ in range(reps):
+                for f in frames:
+                    rc, _ = sf.find(f)
+                    if rc != 0:
+                        raise RuntimeError(f"plain host-fed: status {rc}")
+                    count += 1
+            out[name] = count / (time.perf_counter() - t0)
+    finally:
+        for p, _ in pinned:
+            sf.lib().sf_host_free(p)
+    return {"value": out["pinned"], "unit": "frames/s",
+            "pageable_frames_per_s": out["pageable"],
+            "workload": "gpu::find on 16M uint32 host frames: upload, fused "
+                        "threshold, ordering, list copied back, one frame at a "
+                        "time, unlabelled; value with the frames in pinned memory"}
 
 
 def serving(sf, seconds, seed):
@@ -602,6 +677,8 @@ def main():
         ws.close()
         line["plain_per_frame"] = {"value": plain, "unit": "frames/s",
                                    "labelled": label}
+        line["plain_per_frame_h2d"] = plain_host_fed(sf, frames_host)
+        log(f"plain per frame, host-fed: {line['plain_per_frame_h2d']}")
 
     if not args.no_uneven:
         line["uneven_batch"] = uneven_batch(sf, reps=5, graph=bool(args.graph))
@@ -622,7 +699,7 @@ def main():
         line["refl"] = refl(sf)
         log(f"refl: {line['refl']}")
     if not args.no_cpu:
-        line["cpu_baseline"] = cpu_baseline(frames_host, min(16, os.cpu_count() or 1))
+        line["cpu_baseline"] = cpu_baseline(frames_host)
     print(json.dumps(line), flush=True)
 
 

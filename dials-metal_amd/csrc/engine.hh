@@ -80,6 +80,8 @@ public:
                     bool fresh = true);
   // Device bytes a slot of this shape holds once labelled.
   static size_t slot_bytes(uint32_t height, uint32_t width, int bits);
+  // Device bytes the slots hold now (their buffers only grow).
+  size_t device_bytes() const;
   // Copies a labelled slot's labels, components, frame links and, when
   // `list` is given, its signal pixels (n of each), and waits unless `wait`
   // is false (the caller then waits on stream()).

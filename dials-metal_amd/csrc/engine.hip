@@ -158,6 +158,20 @@ size_t Engine::slot_bytes(uint32_t height, uint32_t width, int bits) {
          (1u << 16);
 }
 
+size_t Engine::device_bytes() const {
+  size_t total = counters_.bytes + table_.bytes + desc_.bytes + queue_.bytes +
+                 scr0_.bytes + scr1_.bytes;
+  for (const auto &p : slots_) {
+    const FrameSlot &s = *p;
+    for (const DeviceBuffer *b :
+         {&s.mask, &s.prefix, &s.totals, &s.unordered, &s.ordered, &s.parent,
+          &s.roots, &s.root_prefix, &s.comp_of, &s.comps, &s.links,
+          &s.row_runs, &s.comp_runs})
+      total += b->bytes;
+  }
+  return total;
+}
+
 void Engine::shape_label(FrameSlot &s) {
   const size_t cap = s.capacity;
   s.root_words = cap / 32 + 2;

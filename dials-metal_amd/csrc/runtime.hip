@@ -138,10 +138,9 @@ struct sf_workspace {
   bool has_frame() { return engine.slot_count() > 0 && slot().thresholded; }
 };
 
-// A batch: an engine whose slots hold each batch in a contiguous range that
-// leaves the batch before it untouched (its last frame links the next batch's
-// first): from slot 0 when the batch fits below the previous one's range,
-// else just above it.
+// A batch: an engine whose slots hold each batch in slots 0..n-1, but for
+// the previous batch's last frame, whose slot is left alone (it links the
+// next batch's first frame): the frame that would take it goes to slot n.
 struct sf_batch {
   explicit sf_batch(void *stream) : engine(stream) {}
   Engine engine;
@@ -697,6 +696,10 @@ int sf_batch_create(void *stream, sf_batch **out) {
 
 void sf_batch_destroy(sf_batch *b) { delete b; }
 
+size_t sf_batch_device_bytes(const sf_batch *b) {
+  return b ? b->engine.device_bytes() : 0;
+}
+
 void *sf_batch_stream(sf_batch *b) { return b ? b->engine.stream() : nullptr; }
 
 int sf_batch_threshold(sf_batch *b, const sf_frame_desc *frames, size_t n,
@@ -709,14 +712,15 @@ int sf_batch_threshold(sf_batch *b, const sf_frame_desc *frames, size_t n,
         (frames[i].bits != 16 && frames[i].bits != 32))
       return bad("a frame's pointer, shape or bits");
   return guarded([&]() -> int {
+    // Slots 0..n-1, except that the previous batch's last frame keeps its
+    // slot (this batch's first frame links to it): the frame that would
+    // land there takes slot n.  A batch never needs more than n + 1 slots.
     const int prev_last = b->ids.empty() ? -1 : b->ids.back();
-    const size_t prev_first = b->ids.empty() ? 0 : static_cast<size_t>(b->ids[0]);
-    const size_t first =
-        b->ids.empty() || n <= prev_first ? 0 : static_cast<size_t>(prev_last) + 1;
     b->ids.resize(n);
     std::vector<FrameIn> in(n);
     for (size_t i = 0; i < n; i++) {
-      b->ids[i] = static_cast<int>(first + i);
+      b->ids[i] = static_cast<int>(i) == prev_last ? static_cast<int>(n)
+                                                   : static_cast<int>(i);
       in[i] = FrameIn{frames[i].frame, frames[i].height, frames[i].width,
                       frames[i].bits};
     }

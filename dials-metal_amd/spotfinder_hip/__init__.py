@@ -198,6 +198,7 @@ def lib():
                             ctypes.POINTER(sz)], ctypes.c_int),
         "sf_inject_find_failure": ([ctypes.c_uint64], None),
         "sf_find_idle": ([], ctypes.c_int),
+        "sf_batch_device_bytes": ([vp], sz),
         "sf_threshold_noalloc": ([ctypes.c_int, vp, u32, u32, vp, sz,
                                   ctypes.POINTER(sz), vp], ctypes.c_int),
         "sf_noalloc_scratch_bytes": ([], sz),
@@ -630,6 +631,9 @@ class Batch:
         a, b = ctypes.c_double(0), ctypes.c_double(0)
         _check(lib().sf_batch_kernel_ms(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+    def device_bytes(self):
+        return int(lib().sf_batch_device_bytes(self.h))
 
     def fallbacks(self):
         return int(lib().sf_batch_fallbacks(self.h))

@@ -266,9 +266,10 @@ const uint32_t *sf_ws_device_mask(sf_workspace *ws, uint32_t *words_per_row);
  * each frame's signal pixels ordered, and the frames labelled, in a fixed
  * number of launches whatever the batch size.  With use_graph the sequence
  * is captured into a hipGraph the first time a batch of these shapes is seen
- * and replayed after.  Two batches alternate in the batch's buffers, so the
- * previous batch stays readable while the next one runs, and frame 0 of a
- * batch can be linked to the last frame of the one before.
+ * and replayed after.  A batch of n frames takes the buffers of n + 1 frames
+ * at most: the previous batch's last frame is kept, so that frame 0 of a
+ * batch can be linked to it; the rest of the previous batch is overwritten
+ * (read it before the next sf_batch_threshold).
  * ------------------------------------------------------------------------ */
 
 typedef struct sf_frame_desc {
@@ -281,6 +282,8 @@ typedef struct sf_batch sf_batch;
 int sf_batch_create(void *stream, sf_batch **out);
 void sf_batch_destroy(sf_batch *b);
 void *sf_batch_stream(sf_batch *b);
+/* Device bytes the batch's frame buffers and scratch hold (they only grow). */
+size_t sf_batch_device_bytes(const sf_batch *b);
 /* Enqueues; returns at once.  At most 4096 frames. */
 int sf_batch_threshold(sf_batch *b, const sf_frame_desc *frames, size_t n,
                        int use_graph);

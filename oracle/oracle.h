@@ -90,12 +90,13 @@ const oracle_pixel *oracle_labeller_pixels(const oracle_labeller *l);
 oracle_counts oracle_labeller_counts(const oracle_labeller *l);
 size_t oracle_labeller_live(const oracle_labeller *l);
 
-/* The CPU baseline: `frames` frames of height*width, thresholded by `threads`
+/* The CPU baseline: `total` frames of height*width taken in turn from the
+ * `nframes` given (frame k is frames[k % nframes]), thresholded by `threads`
  * workers, one frame per worker at a time.  Returns total signal pixels and the
  * wall time in seconds through *seconds. */
 uint64_t oracle_dext_u32_threads(const uint32_t *frames, size_t nframes,
-                                 size_t height, size_t width, int threads,
-                                 double *seconds);
+                                 size_t total, size_t height, size_t width,
+                                 int threads, double *seconds);
 
 /* Decompression (oracle/decompress.c).  oracle_lz4_decode_block: bytes
  * written, or < 0 for a malformed block.  oracle_bslz4_decode: bytes consumed,

@@ -73,7 +73,7 @@ class _Lib:
         getattr(L, f"{P}labeller_live").argtypes = [vp]
         getattr(L, f"{P}labeller_live").restype = sz
         f = getattr(L, f"{P}dext_u32_threads")
-        f.argtypes = [vp, sz, sz, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
+        f.argtypes = [vp, sz, sz, sz, sz, ctypes.c_int, ctypes.POINTER(ctypes.c_double)]
         f.restype = ctypes.c_uint64
         if prefix == "oracle_":
             L.oracle_labeller_spots.argtypes = [vp]
@@ -130,12 +130,15 @@ class _Lib:
         getattr(self.lib, f"{self.prefix}sort_by_index")(_p(pixels), len(pixels))
         return pixels
 
-    def threads_u32(self, frames, threads):
+    def threads_u32(self, frames, threads, total=None):
+        """`total` frames (default: each once) taken in turn from `frames`,
+        on `threads` workers: (signal pixels, wall seconds)."""
         frames = np.ascontiguousarray(frames, np.uint32)
         n, h, w = frames.shape
         secs = ctypes.c_double(0)
         sig = getattr(self.lib, f"{self.prefix}dext_u32_threads")(
-            _p(frames), n, h, w, threads, ctypes.byref(secs))
+            _p(frames), n, n if total is None else int(total), h, w, threads,
+            ctypes.byref(secs))
         return int(sig), secs.value
 
     # ---- decompression (oracle only: decompress.cc needs the absent

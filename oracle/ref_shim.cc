@@ -153,8 +153,8 @@ int ref_refl_write(const char *path, const dials_spots::Spot *spots,
 // The reference's threshold on N threads, one frame each at a time, with a
 // per-thread scratch as mxi_find keeps one (find_spots.cc:404-405).
 std::uint64_t ref_dext_u32_threads(const std::uint32_t *frames,
-                                   std::size_t nframes, std::size_t h,
-                                   std::size_t w, int threads,
+                                   std::size_t nframes, std::size_t total,
+                                   std::size_t h, std::size_t w, int threads,
                                    double *seconds) {
   std::atomic<std::size_t> next{0};
   std::atomic<std::uint64_t> signal{0};
@@ -168,9 +168,9 @@ std::uint64_t ref_dext_u32_threads(const std::uint32_t *frames,
       std::vector<SignalPixel> out;
       for (;;) {
         const std::size_t f = next.fetch_add(1);
-        if (f >= nframes)
+        if (f >= total)
           break;
-        dext<std::uint32_t>(frames + f * h * w, out, h, w, scratch);
+        dext<std::uint32_t>(frames + (f % nframes) * h * w, out, h, w, scratch);
         signal.fetch_add(out.size());
       }
     });

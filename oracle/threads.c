@@ -13,7 +13,7 @@
 
 typedef struct {
   const uint32_t *frames;
-  size_t nframes, height, width;
+  size_t nframes, total, height, width;
   atomic_size_t next;
   atomic_uint_fast64_t signal;
 } job;
@@ -26,11 +26,11 @@ static void *worker(void *arg) {
       (oracle_signal_pixel *)malloc(sizeof(oracle_signal_pixel) * capacity);
   for (;;) {
     const size_t f = atomic_fetch_add(&j->next, 1);
-    if (f >= j->nframes)
+    if (f >= j->total)
       break;
     size_t found = 0;
-    oracle_dext_u32(j->frames + f * n, j->height, j->width, out, capacity,
-                    &found);
+    oracle_dext_u32(j->frames + (f % j->nframes) * n, j->height, j->width, out,
+                    capacity, &found);
     atomic_fetch_add(&j->signal, (uint_fast64_t)found);
   }
   free(out);
@@ -38,11 +38,12 @@ static void *worker(void *arg) {
 }
 
 uint64_t oracle_dext_u32_threads(const uint32_t *frames, size_t nframes,
-                                 size_t height, size_t width, int threads,
-                                 double *seconds) {
+                                 size_t total, size_t height, size_t width,
+                                 int threads, double *seconds) {
   job j;
   j.frames = frames;
   j.nframes = nframes;
+  j.total = total;
   j.height = height;
   j.width = width;
   atomic_init(&j.next, 0);

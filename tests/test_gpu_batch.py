@@ -181,3 +181,43 @@ def test_autotune_installs_a_shape_and_results_do_not_change(device):
             assert batch.fetch(i, counts2[i]).tobytes() == before[i].tobytes(), i
     finally:
         device.set_tuning(**start)
+
+
+def test_batch_size_ramp_keeps_slots_bounded(device):
+    """Batches of 1, 2, ..., 8 frames in turn: each links its first frame to
+    the previous batch's last, the grouping across all of them equals the
+    oracle's, and the batch never holds more than the buffers of the largest
+    batch plus one frame (measured against a fresh batch of nine)."""
+    desc = device.synth_desc((640, 480), spots_per_frame=60, persist=1, seed=71)
+    h, w = desc.height, desc.width
+    total = sum(range(1, 9))
+    host = [device.synth_frame(desc, z) for z in range(total)]
+    dev = [device.DeviceArray.from_numpy(f) for f in host]
+    lab = device.Labeller(h, w)
+    ref = O.oracle().labeller(h, w)
+    batch = device.Batch()
+    z = 0
+    for n in range(1, 9):
+        say("ramp batch", n)
+        batch.threshold([(dev[k].ptr, h, w, 32) for k in range(z, z + n)], graph=(n % 2 == 0))
+        counts, status = batch.sync()
+        assert status == [0] * n
+        batch.label(True)
+        lab.add_batch(z, batch)
+        z += n
+    ramp_bytes = batch.device_bytes()
+    batch.close()
+    for k in range(total):
+        ref.add(k, O.oracle().dext(host[k])[1])
+    lab.finish()
+    ref.finish()
+    compare_spots(lab.spots(), ref.spots())
+    assert lab.pixels().tobytes() == ref.pixels().tobytes()
+    assert lab.counts() == ref.counts()
+    nine = device.Batch()
+    nine.threshold([(dev[k].ptr, h, w, 32) for k in range(9)])
+    nine.sync()
+    nine.label(True)
+    nine.labels(0)
+    assert ramp_bytes <= nine.device_bytes(), (ramp_bytes, nine.device_bytes())
+    nine.close()
