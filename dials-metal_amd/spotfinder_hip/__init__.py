@@ -17,7 +17,10 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIBRARY = os.path.join(HERE, "libspotfinder_hip.so")
+# SPOTFINDER_HIP_LIBRARY names another build of the same library (the
+# sanitizer build of `make -C dials-metal_amd check-sanitize`, for one).
+LIBRARY = os.environ.get("SPOTFINDER_HIP_LIBRARY") or os.path.join(
+    HERE, "libspotfinder_hip.so")
 
 SIGNAL_PIXEL = np.dtype(
     [("index", "<u4"), ("value", "<u4"), ("background", "<f4"),
