@@ -917,6 +917,16 @@ class Decoder:
         _check(rc)
         self.n = len(chunks)
 
+    def load_descriptors(self, arr, n, keep=None):
+        """Loads n sf_chunk descriptors as they stand (NxmxSeries.read_chunks'
+        output); `keep` holds whatever their data point into."""
+        rc = lib().sf_decoder_load(self.h, arr, n)
+        if rc == -1:
+            raise DecompressError(last_error())
+        _check(rc)
+        self._keep = keep
+        self.n = n
+
     def run(self):
         _check(lib().sf_decoder_run(self.h))
 
@@ -936,6 +946,149 @@ class Decoder:
         ms = ctypes.c_double(0)
         _check(lib().sf_decoder_kernel_ms(self.h, ctypes.byref(ms)))
         return ms.value
+
+
+# ---- the NXmx chunk reader (libspotfinder_nxmx.so) ----
+
+NXMX_LIBRARY = os.path.join(HERE, "libspotfinder_nxmx.so")
+_nxmx = None
+
+
+class NxmxInfo(ctypes.Structure):
+    _fields_ = [("images", ctypes.c_uint64), ("height", ctypes.c_uint64),
+                ("width", ctypes.c_uint64)]
+
+
+class NxmxBlock(ctypes.Structure):
+    _fields_ = [("filename", ctypes.c_char_p), ("dataset", ctypes.c_char_p),
+                ("first", ctypes.c_uint64), ("last", ctypes.c_uint64),
+                ("offset", ctypes.c_uint64)]
+
+
+class NxmxFrame(ctypes.Structure):
+    _fields_ = [("number", ctypes.c_uint64), ("height", ctypes.c_uint64),
+                ("width", ctypes.c_uint64), ("bit_depth", ctypes.c_uint32),
+                ("algorithm", ctypes.c_char_p), ("size", ctypes.c_size_t)]
+
+    def as_dict(self):
+        return {"number": self.number, "height": self.height, "width": self.width,
+                "bit_depth": self.bit_depth,
+                "algorithm": (self.algorithm or b"").decode(), "size": self.size}
+
+
+class NxmxError(RuntimeError):
+    """What series::nxmx throws (std::runtime_error), through the C ABI."""
+
+
+def nxmx_lib():
+    """libspotfinder_nxmx.so, the reader over HDF5 (built where HDF5 is)."""
+    global _nxmx
+    if _nxmx is not None:
+        return _nxmx
+    if not os.path.exists(NXMX_LIBRARY):
+        raise ImportError(f"{NXMX_LIBRARY} is not built (needs the HDF5 headers)")
+    L = ctypes.CDLL(NXMX_LIBRARY)
+    vp, sz, u64 = ctypes.c_void_p, ctypes.c_size_t, ctypes.c_uint64
+    for name, args, res in (
+            ("sf_nxmx_open", [ctypes.c_char_p, ctypes.POINTER(vp),
+                              ctypes.POINTER(NxmxInfo)], ctypes.c_int),
+            ("sf_nxmx_close", [vp], None),
+            ("sf_nxmx_block_count", [vp], sz),
+            ("sf_nxmx_get_block", [vp, sz, ctypes.POINTER(NxmxBlock)], ctypes.c_int),
+            ("sf_nxmx_frame_info", [vp, u64, ctypes.POINTER(NxmxFrame)], ctypes.c_int),
+            ("sf_nxmx_read", [vp, u64, vp, sz, ctypes.POINTER(NxmxFrame)], ctypes.c_int),
+            ("sf_nxmx_read_chunks", [vp, u64, sz, vp, sz, vp, vp,
+                                     ctypes.POINTER(sz), ctypes.POINTER(sz), vp],
+             ctypes.c_int),
+            ("sf_nxmx_last_error", [], ctypes.c_char_p)):
+        f = getattr(L, name)
+        f.argtypes, f.restype = args, res
+    _nxmx = L
+    return L
+
+
+class NxmxSeries:
+    """series::nxmx (src/spots/series.hh:32-97): an NXmx master file's frames
+    as raw chunks.  Raises FileNotFoundError while the master is not a
+    regular file, NxmxError for anything the file or HDF5 refuses."""
+
+    def __init__(self, master):
+        L = nxmx_lib()
+        self.h = ctypes.c_void_p()
+        info = NxmxInfo()
+        rc = L.sf_nxmx_open(str(master).encode(), ctypes.byref(self.h), ctypes.byref(info))
+        if rc == 1:
+            raise FileNotFoundError(str(master))
+        self._status(rc)
+        self.images, self.height, self.width = info.images, info.height, info.width
+
+    def _status(self, rc):
+        if rc == -1:
+            raise ValueError(nxmx_lib().sf_nxmx_last_error().decode())
+        if rc == -3:
+            raise NxmxError(nxmx_lib().sf_nxmx_last_error().decode())
+
+    def close(self):
+        if self.h:
+            nxmx_lib().sf_nxmx_close(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def blocks(self):
+        L = nxmx_lib()
+        out = []
+        for i in range(L.sf_nxmx_block_count(self.h)):
+            b = NxmxBlock()
+            self._status(L.sf_nxmx_get_block(self.h, i, ctypes.byref(b)))
+            out.append({"filename": b.filename.decode(), "dataset": b.dataset.decode(),
+                        "first": b.first, "last": b.last, "offset": b.offset})
+        return out
+
+    def frame_info(self, frame):
+        f = NxmxFrame()
+        rc = nxmx_lib().sf_nxmx_frame_info(self.h, frame, ctypes.byref(f))
+        self._status(rc)
+        return None if rc == 1 else f.as_dict()
+
+    def read(self, frame):
+        """Reader::read: (chunk bytes, description), or None for a frame
+        never written."""
+        f = NxmxFrame()
+        L = nxmx_lib()
+        rc = L.sf_nxmx_frame_info(self.h, frame, ctypes.byref(f))
+        self._status(rc)
+        if rc == 1:
+            return None
+        buf = ctypes.create_string_buffer(max(1, f.size))
+        rc = L.sf_nxmx_read(self.h, frame, buf, f.size, ctypes.byref(f))
+        self._status(rc)
+        return buf.raw[: f.size], f.as_dict()
+
+    def read_chunks(self, first, frames):
+        """Frames first .. first + len(frames) - 1 (frames: their device
+        frames, DeviceArray or address) read into one host buffer: (Chunk
+        array, count, buffer, present flags) for Decoder.load_descriptors."""
+        L = nxmx_lib()
+        n = len(frames)
+        ptrs = (ctypes.c_void_p * max(n, 1))(
+            *[f.ptr if hasattr(f, "ptr") else f for f in frames])
+        chunks = (Chunk * max(n, 1))()
+        present = (ctypes.c_int * max(n, 1))()
+        loaded, used = ctypes.c_size_t(0), ctypes.c_size_t(0)
+        rc = L.sf_nxmx_read_chunks(self.h, first, n, None, 0, ptrs, chunks,
+                                   ctypes.byref(loaded), ctypes.byref(used), present)
+        if rc != -2:
+            self._status(rc)
+        buf = (ctypes.c_uint8 * max(1, used.value))()
+        rc = L.sf_nxmx_read_chunks(self.h, first, n, buf, used.value, ptrs, chunks,
+                                   ctypes.byref(loaded), ctypes.byref(used), present)
+        self._status(rc)
+        return chunks, loaded.value, buf, list(present)[:n]
 
 
 # ---- --gpu-force and mxi_max on frames in HBM ----

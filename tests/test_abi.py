@@ -10,9 +10,12 @@ import numpy as np
 from conftest import ROOT
 
 
-def declared_functions():
+def declared_functions(headers=None):
     names = set()
-    for h in glob.glob(os.path.join(ROOT, "include", "*.h")):
+    paths = ([os.path.join(ROOT, "include", h) for h in headers] if headers else
+             [h for h in glob.glob(os.path.join(ROOT, "include", "*.h"))
+              if not h.endswith("spotfinder_nxmx.h")])
+    for h in paths:
         text = open(h).read()
         text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
         for m in re.finditer(r"\b(sf_[a-z0-9_]+)\s*\(", text):
@@ -30,6 +33,19 @@ def test_headers_declare_the_surface():
 def test_library_exports_every_declared_function(sf):
     lib = ctypes.CDLL(sf.LIBRARY)
     missing = [n for n in sorted(declared_functions()) if not hasattr(lib, n)]
+    assert not missing, missing
+
+
+def test_reader_library_exports_its_header(sf):
+    """libspotfinder_nxmx.so (built where HDF5 is) exports every function
+    include/spotfinder_nxmx.h declares."""
+    import pytest
+    if not os.path.exists(sf.NXMX_LIBRARY):
+        pytest.skip("the NXmx reader is not built (no HDF5 headers)")
+    lib = ctypes.CDLL(sf.NXMX_LIBRARY)
+    names = declared_functions(["spotfinder_nxmx.h"])
+    assert "sf_nxmx_read_chunks" in names
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
 
 
