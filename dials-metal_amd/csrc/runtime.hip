@@ -20,7 +20,7 @@
 #include <vector>
 
 #include "engine.hh"
-#include "group_device.hh"
+#include "stitch.hh"
 #include "sf_internal.hh"
 #include "sf_labeller.hh"
 #include "spotfinder_hip.h"
@@ -233,8 +233,8 @@ struct sf_labeller {
   // labelled batch.
   void add_slot(int64_t frame, Engine &e, int id,
                 const sf_signal_pixel *host_list, bool fresh = true) {
-    if (pixels_on_device) {
-      add_device(frame, e, &id, 1, fresh);
+    if (ds) {
+      add_device(frame, e, &id, 1);
       return;
     }
     size_t n = 0, ncomp = 0, nlinks = 0;
@@ -253,73 +253,38 @@ struct sf_labeller {
     last_serial = s.serial;
   }
 
-  // Device mode: the pixels stay on the device (DevicePixels); only the
-  // components and links come back.  Slots ids[0..count) become frames
-  // first_frame onward.
-  std::unique_ptr<DevicePixels> dp;
-  bool pixels_on_device = false;
-  std::vector<size_t> n_, ncomp_, nlinks_, at_comp_, at_link_;
-  std::vector<const void *> frame_pixels_;
+  // Device mode: the grouping itself runs on the device (DeviceStitch); slots
+  // ids[0..count) become frames first_frame onward.  A frame's links count
+  // only if they were taken against the frame added just before it.
+  std::unique_ptr<DeviceStitch> ds;
 
-  void add_device(int64_t first_frame, Engine &e, const int *ids, size_t count,
-                  bool fresh) {
-    n_.resize(count);
-    ncomp_.resize(count);
-    nlinks_.resize(count);
-    at_comp_.resize(count + 1);
-    at_link_.resize(count + 1);
-    at_comp_[0] = at_link_[0] = 0;
+  void check_order(int64_t frame) const {
+    if (ds && ds->started() && frame <= ds->last_frame())
+      throw std::runtime_error("dials_spots: frame " + std::to_string(frame) +
+                               " arrived after frame " +
+                               std::to_string(ds->last_frame()) +
+                               "; frames must be added in ascending order");
+  }
+
+  void add_device(int64_t first_frame, Engine &e, const int *ids, size_t count) {
+    check_order(first_frame);
+    std::unique_ptr<bool[]> linked(new bool[count]);
+    const Engine *engine = last_engine;
+    uint64_t serial = last_serial;
     for (size_t i = 0; i < count; i++) {
-      e.label_counts(ids[i], &n_[i], &ncomp_[i], &nlinks_[i], fresh && i == 0);
-      at_comp_[i + 1] = at_comp_[i] + ncomp_[i];
-      at_link_[i + 1] = at_link_[i] + nlinks_[i];
+      const FrameSlot &s = e.slot(ids[i]);
+      linked[i] = s.linked_serial != 0 && (i > 0 || ds->started()) &&
+                  engine == &e && serial == s.linked_serial;
+      engine = &e;
+      serial = s.serial;
     }
-    sf_component *cs = comps.get<sf_component>(at_comp_[count]);
-    uint32_t *ln = links.get<uint32_t>(2 * at_link_[count]);
-    for (size_t i = 0; i < count; i++)
-      e.fetch_labels(ids[i], nullptr, 0, cs + at_comp_[i], ncomp_[i],
-                     ln + 2 * at_link_[i], nlinks_[i], nullptr, false);
-    uint64_t owner = 0;
-    frame_pixels_.resize(count);
-    dp->stage(e, ids, count, n_.data(), ncomp_.data(), &owner,
-              frame_pixels_.data());
-    check(hipStreamSynchronize(e.stream()), "hipStreamSynchronize");
-    size_t added = 0;
-    try {
-      for (size_t i = 0; i < count; i++) {
-        const FrameSlot &s = e.slot(ids[i]);
-        const bool to_previous = s.linked_serial != 0 && lab.started() &&
-                                 last_engine == &e &&
-                                 last_serial == s.linked_serial;
-        lab.add_device(first_frame + static_cast<int64_t>(i), frame_pixels_[i],
-                       owner, n_[i], cs + at_comp_[i], ncomp_[i],
-                       ln + 2 * at_link_[i], nlinks_[i], to_previous);
-        added++;
-        last_engine = &e;
-        last_serial = s.serial;
-      }
-    } catch (...) {
-      // Frames the stitch never took are not its to release.
-      for (size_t i = added; i < count; i++)
-        dp->release(owner);
-      throw;
-    }
-    drain();
+    ds->add(e, ids, count, first_frame, linked.get());
+    last_engine = &e;
+    last_serial = e.slot(ids[count - 1]).serial;
   }
 
-  // Writes the spots emitted so far into the pixel list and returns the
-  // frames the stitch released to the pool.
-  void drain() {
-    const size_t have = lab.pixels().size(), total = lab.kept_total();
-    if (total > have) {
-      sf_pixel *dest = lab.pixels_mut().extend(total - have);
-      dp->gather(lab.tasks(), have, total - have, dest);
-    }
-    lab.tasks().clear();
-    for (const uint64_t owner : lab.released())
-      dp->release(owner);
-    lab.released().clear();
-  }
+  const std::vector<sf_spot> *dev_spots() const { return ds ? &ds->spots() : nullptr; }
+  const std::vector<sf_pixel> *dev_pixels() const { return ds ? &ds->pixels() : nullptr; }
 };
 
 extern "C" {
@@ -963,9 +928,8 @@ int sf_labeller_create_device(size_t height, size_t width,
     return rc;
   return guarded([&]() -> int {
     try {
-      (*out)->lab.use_device_pixels();
-      (*out)->dp = std::make_unique<DevicePixels>();
-      (*out)->pixels_on_device = true;
+      (*out)->lab.use_device_pixels(); // refuses host-labelled frames
+      (*out)->ds = std::make_unique<DeviceStitch>(height, width, (*out)->lab.options());
       return 0;
     } catch (...) {
       delete *out;
@@ -976,7 +940,7 @@ int sf_labeller_create_device(size_t height, size_t width,
 }
 
 size_t sf_labeller_device_bytes(const sf_labeller *l) {
-  return l && l->dp ? l->dp->held_bytes() : 0;
+  return l && l->ds ? l->ds->held_bytes() : 0;
 }
 
 void sf_labeller_destroy(sf_labeller *l) { delete l; }
@@ -987,6 +951,7 @@ int sf_labeller_add(sf_labeller *l, int64_t frame,
     return bad("labeller or list");
   try {
     l->lab.check_frame(frame, signal, n);
+    l->check_order(frame);
   } catch (const std::exception &e) {
     set_last_error(e.what());
     return -1;
@@ -1044,8 +1009,8 @@ int sf_labeller_add_batch(sf_labeller *l, int64_t first_frame, sf_batch *b) {
                  "labeller's shape");
   }
   return add_checked(l, [&] {
-    if (l->pixels_on_device) {
-      l->add_device(first_frame, b->engine, b->ids.data(), b->ids.size(), true);
+    if (l->ds) {
+      l->add_device(first_frame, b->engine, b->ids.data(), b->ids.size());
       return;
     }
     for (size_t i = 0; i < b->ids.size(); i++)
@@ -1074,35 +1039,77 @@ int sf_labeller_finish(sf_labeller *l) {
   if (!l)
     return bad("labeller");
   return add_checked(l, [&] {
-    l->lab.finish();
-    if (l->pixels_on_device)
-      l->drain();
+    if (l->ds)
+      l->ds->finish();
+    else
+      l->lab.finish();
   });
 }
 
+// In device mode the records are copied from the device on first use after
+// a change; a device failure then reads as no records (sf_last_error says).
+extern "C++" {
+namespace {
+template <typename F> auto or_empty(F &&f) -> decltype(f()) {
+  try {
+    return f();
+  } catch (const std::exception &e) {
+    set_last_error(e.what());
+    return nullptr;
+  }
+}
+} // namespace
+}
+
 size_t sf_labeller_spot_count(const sf_labeller *l) {
-  return l ? l->lab.spots().size() : 0;
+  if (!l)
+    return 0;
+  if (!l->ds)
+    return l->lab.spots().size();
+  const auto *v = or_empty([&] { return l->dev_spots(); });
+  return v ? v->size() : 0;
 }
 const sf_spot *sf_labeller_spots(const sf_labeller *l) {
-  return l ? l->lab.spots().data() : nullptr;
+  if (!l)
+    return nullptr;
+  if (!l->ds)
+    return l->lab.spots().data();
+  const auto *v = or_empty([&] { return l->dev_spots(); });
+  return v ? v->data() : nullptr;
 }
 size_t sf_labeller_pixel_count(const sf_labeller *l) {
-  return l ? l->lab.pixels().size() : 0;
+  if (!l)
+    return 0;
+  if (!l->ds)
+    return l->lab.pixels().size();
+  const auto *v = or_empty([&] { return l->dev_pixels(); });
+  return v ? v->size() : 0;
 }
 const sf_pixel *sf_labeller_pixels(const sf_labeller *l) {
-  return l ? l->lab.pixels().data() : nullptr;
+  if (!l)
+    return nullptr;
+  if (!l->ds)
+    return l->lab.pixels().data();
+  const auto *v = or_empty([&] { return l->dev_pixels(); });
+  return v ? v->data() : nullptr;
 }
 sf_counts sf_labeller_counts(const sf_labeller *l) {
-  return l ? l->lab.counts() : sf_counts{};
+  if (!l)
+    return sf_counts{};
+  return l->ds ? l->ds->counts() : l->lab.counts();
 }
-size_t sf_labeller_live(const sf_labeller *l) { return l ? l->lab.live() : 0; }
+size_t sf_labeller_live(const sf_labeller *l) {
+  if (!l)
+    return 0;
+  return l->ds ? l->ds->live_pixels() : l->lab.live();
+}
 
 int sf_labeller_write_refl(const sf_labeller *l, const char *path,
                            const sf_refl_options *options) {
   if (l == nullptr)
     return bad("labeller");
-  return sf_refl_write(path, l->lab.spots().data(), l->lab.spots().size(),
-                       l->lab.pixels().data(), l->lab.pixels().size(),
+  return sf_refl_write(path, sf_labeller_spots(l), sf_labeller_spot_count(l),
+                       sf_labeller_pixels(l), sf_labeller_pixel_count(l),
                        l->lab.width(), options);
 }
 
