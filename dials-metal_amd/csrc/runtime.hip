@@ -283,8 +283,7 @@ struct sf_labeller {
     last_serial = e.slot(ids[count - 1]).serial;
   }
 
-  const std::vector<sf_spot> *dev_spots() const { return ds ? &ds->spots() : nullptr; }
-  const std::vector<sf_pixel> *dev_pixels() const { return ds ? &ds->pixels() : nullptr; }
+
 };
 
 extern "C" {
@@ -1046,16 +1045,16 @@ int sf_labeller_finish(sf_labeller *l) {
   });
 }
 
-// In device mode the records are copied from the device on first use after
-// a change; a device failure then reads as no records (sf_last_error says).
+// In device mode the records come from the device grouping's host copies; a
+// device failure then reads as no records (sf_last_error says).
 extern "C++" {
 namespace {
-template <typename F> auto or_empty(F &&f) -> decltype(f()) {
+template <typename F> auto records_of(F &&f) -> decltype(f()) {
   try {
     return f();
   } catch (const std::exception &e) {
     set_last_error(e.what());
-    return nullptr;
+    return {nullptr, 0};
   }
 }
 } // namespace
@@ -1066,32 +1065,28 @@ size_t sf_labeller_spot_count(const sf_labeller *l) {
     return 0;
   if (!l->ds)
     return l->lab.spots().size();
-  const auto *v = or_empty([&] { return l->dev_spots(); });
-  return v ? v->size() : 0;
+  return records_of([&] { return l->ds->spots(); }).size;
 }
 const sf_spot *sf_labeller_spots(const sf_labeller *l) {
   if (!l)
     return nullptr;
   if (!l->ds)
     return l->lab.spots().data();
-  const auto *v = or_empty([&] { return l->dev_spots(); });
-  return v ? v->data() : nullptr;
+  return records_of([&] { return l->ds->spots(); }).data;
 }
 size_t sf_labeller_pixel_count(const sf_labeller *l) {
   if (!l)
     return 0;
   if (!l->ds)
     return l->lab.pixels().size();
-  const auto *v = or_empty([&] { return l->dev_pixels(); });
-  return v ? v->size() : 0;
+  return records_of([&] { return l->ds->pixels(); }).size;
 }
 const sf_pixel *sf_labeller_pixels(const sf_labeller *l) {
   if (!l)
     return nullptr;
   if (!l->ds)
     return l->lab.pixels().data();
-  const auto *v = or_empty([&] { return l->dev_pixels(); });
-  return v ? v->data() : nullptr;
+  return records_of([&] { return l->ds->pixels(); }).data;
 }
 sf_counts sf_labeller_counts(const sf_labeller *l) {
   if (!l)

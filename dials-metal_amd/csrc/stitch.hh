@@ -9,7 +9,9 @@
 #include <hip/hip_runtime.h>
 
 #include <cstddef>
+#include <algorithm>
 #include <cstdint>
+#include <cstring>
 #include <memory>
 #include <vector>
 
@@ -18,6 +20,12 @@
 #include "sf_labeller.hh"
 
 namespace sfi {
+
+// Records and their count.
+template <typename T> struct Records {
+  const T *data;
+  size_t size;
+};
 
 class DeviceStitch {
 public:
@@ -38,10 +46,10 @@ public:
   bool started() const { return started_; }
   int64_t last_frame() const { return last_frame_; }
 
-  // Host copies of the records emitted so far (copied from the device on
-  // first use after a change).
-  const std::vector<sf_spot> &spots();
-  const std::vector<sf_pixel> &pixels();
+  // The records emitted so far, in host memory: each batch's are copied back
+  // while the next batch runs, and these wait for the copies.
+  Records<sf_spot> spots();
+  Records<sf_pixel> pixels();
   sf_counts counts() const { return counts_; }
   size_t live_pixels() const { return live_; }
   // Device bytes of the frames' staged pixels an open spot still needs.
@@ -62,8 +70,6 @@ private:
                size_t new_end, size_t open_begin, size_t open_end,
                size_t last_begin, size_t last_end);
   void grow_table(size_t need);
-  void compact();
-  void release_dead_buffers(const uint32_t *live, size_t n);
 
   hipStream_t stream_ = nullptr;
   size_t height_, width_;
@@ -89,9 +95,9 @@ private:
   std::vector<Staged> staged_;
   std::vector<std::unique_ptr<DeviceBuffer>> spare_;
 
-  size_t spots_dev_ = 0, pixels_dev_ = 0; // records on the device
-  std::vector<sf_spot> spots_host_;
-  std::vector<sf_pixel> pixels_host_;
+  size_t spots_dev_ = 0, pixels_dev_ = 0; // records emitted (and copies queued)
+  sfl::BigArray<sf_spot> spots_host_;
+  sfl::BigArray<sf_pixel> pixels_host_;
   sf_counts counts_{};
   size_t live_ = 0;
 };

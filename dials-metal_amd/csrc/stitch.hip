@@ -500,21 +500,22 @@ __global__ __launch_bounds__(256) void emit_spots(
     sf_spot *__restrict__ spot_tmp, uint32_t *__restrict__ decision,
     uint64_t *__restrict__ accidx, uint64_t *__restrict__ pixoff,
     Counters *__restrict__ counters) {
+  // Every lane reaches the counting at the end; entries 0 .. kmax exist (one
+  // past the largest count, for the scans).
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-  if (k > kmax) // entries 0 .. kmax: one past the largest count, for the scans
-    return;
   const uint32_t K = *bits_prefix_total;
-  if (k >= K) {
+  const bool valid = k < K;
+  if (!valid && k <= kmax) {
     accidx[k] = 0;
     pixoff[k] = 0;
-    return;
   }
-  const Acc a = acc[emit_list[k]];
-  atomicAdd(&counters->groups, 1ull);
-  atomicAdd(&counters->closed_pixels, static_cast<unsigned long long>(a.n));
-  uint32_t d;
+  Acc a{};
+  if (valid)
+    a = acc[emit_list[k]];
+  uint32_t d = 0;
   sf_spot s{};
-  if (a.n < p.min_size) {
+  if (!valid) {
+  } else if (a.n < p.min_size) {
     d = kDecisionSmall;
   } else if (a.n > p.max_size) {
     d = kDecisionLarge;
@@ -584,23 +585,41 @@ __global__ __launch_bounds__(256) void emit_spots(
         d = kDecisionSeparated;
     }
   }
-  decision[k] = d;
-  spot_tmp[k] = s;
-  if (d == kDecisionSmall)
-    atomicAdd(&counters->too_small, 1ull);
-  else if (d == kDecisionLarge)
-    atomicAdd(&counters->too_large, 1ull);
-  else if (d == kDecisionSeparated)
-    atomicAdd(&counters->separated, 1ull);
-  else
-    atomicAdd(&counters->accepted, 1ull);
-  accidx[k] = d == kDecisionAccepted ? 1 : 0;
-  pixoff[k] = d == kDecisionAccepted ? a.n : 0;
+  if (valid) {
+    decision[k] = d;
+    spot_tmp[k] = s;
+    accidx[k] = d == kDecisionAccepted ? 1 : 0;
+    pixoff[k] = d == kDecisionAccepted ? a.n : 0;
+  }
+  // One atomic per counter per wave.
+  uint64_t pix = valid ? a.n : 0;
+  for (int o = 32; o > 0; o >>= 1)
+    pix += __shfl_xor(pix, o);
+  const unsigned long long groups = __popcll(__ballot(valid));
+  const unsigned long long small = __popcll(__ballot(d == kDecisionSmall));
+  const unsigned long long large = __popcll(__ballot(d == kDecisionLarge));
+  const unsigned long long sep = __popcll(__ballot(d == kDecisionSeparated));
+  const unsigned long long acc_n = __popcll(__ballot(d == kDecisionAccepted));
+  if ((threadIdx.x & 63) == 0 && groups) {
+    atomicAdd(&counters->groups, groups);
+    atomicAdd(&counters->closed_pixels, static_cast<unsigned long long>(pix));
+    if (small)
+      atomicAdd(&counters->too_small, small);
+    if (large)
+      atomicAdd(&counters->too_large, large);
+    if (sep)
+      atomicAdd(&counters->separated, sep);
+    if (acc_n)
+      atomicAdd(&counters->accepted, acc_n);
+  }
 }
 
 // Per table node: dead already, dead now (its component closed in this
 // batch; marked), or alive (counted, and against its staged buffer); a node
-// dying now in an accepted spot is a member, counted against its spot.
+// dying now in an accepted spot is a member, counted against its spot.  The
+// grid's lanes cover the nodes in order, so each wave owns two whole words
+// of the member and alive bitmaps and writes them without atomics; counts
+// go in one atomic per wave where the wave agrees.
 __global__ __launch_bounds__(256) void classify_nodes(
     Node *__restrict__ nodes, const Acc *__restrict__ acc, uint32_t end,
     int flush_all, int32_t first, uint64_t T, const uint32_t *__restrict__ bits,
@@ -610,24 +629,49 @@ __global__ __launch_bounds__(256) void classify_nodes(
     uint32_t *__restrict__ live_counts, uint32_t buffer_base, uint32_t nbuffers,
     Summary *__restrict__ summary) {
   const uint32_t x = blockIdx.x * blockDim.x + threadIdx.x;
-  if (x >= end || nodes[x].dead)
-    return;
-  const uint32_t r = root_of(nodes, x);
-  uint64_t bit;
-  if (!closed_bit(nodes, acc, bits, r, flush_all, first, T, &bit)) {
-    atomicOr(&alive_bits[x / 32], 1u << (x % 32));
-    const uint32_t b = nodes[x].buffer - buffer_base;
-    if (b < nbuffers)
-      atomicAdd(&live_counts[b], 1u);
-    return;
+  const uint32_t lane = threadIdx.x & 63;
+  bool alive = false, dies = false, member = false;
+  uint32_t b = 0;
+  if (x < end && !nodes[x].dead) {
+    const uint32_t r = root_of(nodes, x);
+    uint64_t bit;
+    if (!closed_bit(nodes, acc, bits, r, flush_all, first, T, &bit)) {
+      alive = true;
+      b = nodes[x].buffer - buffer_base;
+    } else {
+      dies = true;
+      nodes[x].dead = 1;
+      const uint32_t k = rank_of(bits, prefix, bit);
+      if (decision[k] == kDecisionAccepted) {
+        member = true;
+        atomicAdd(&spot_members[accidx[k]], 1ull);
+      }
+    }
   }
-  nodes[x].dead = 1;
-  atomicAdd(reinterpret_cast<unsigned long long *>(&summary->dead), 1ull);
-  const uint32_t k = rank_of(bits, prefix, bit);
-  if (decision[k] != kDecisionAccepted)
-    return;
-  atomicOr(&membits[x / 32], 1u << (x % 32));
-  atomicAdd(&spot_members[accidx[k]], 1ull);
+  const uint64_t alive_mask = __ballot(alive);
+  const uint64_t member_mask = __ballot(member);
+  const uint32_t w0 = (x - lane) / 32; // the wave's first word (x - lane is 64-aligned)
+  if (lane == 0 && w0 * 32 < end) {
+    alive_bits[w0] = static_cast<uint32_t>(alive_mask);
+    membits[w0] = static_cast<uint32_t>(member_mask);
+  }
+  if (lane == 32 && w0 * 32 + 32 < end) {
+    alive_bits[w0 + 1] = static_cast<uint32_t>(alive_mask >> 32);
+    membits[w0 + 1] = static_cast<uint32_t>(member_mask >> 32);
+  }
+  const unsigned long long died = __popcll(__ballot(dies));
+  if (lane == 0 && died)
+    atomicAdd(reinterpret_cast<unsigned long long *>(&summary->dead), died);
+  if (alive_mask) {
+    // Nodes of one batch share a buffer: usually one atomic for the wave.
+    const int leader = __ffsll(static_cast<unsigned long long>(alive_mask)) - 1;
+    const uint32_t b0 = __shfl(b, leader);
+    const uint64_t same = __ballot(alive && b == b0);
+    if (static_cast<int>(lane) == leader && b0 < nbuffers)
+      atomicAdd(&live_counts[b0], static_cast<uint32_t>(__popcll(same)));
+    if (alive && b != b0 && b < nbuffers)
+      atomicAdd(&live_counts[b], 1u);
+  }
 }
 
 __device__ __forceinline__ uint64_t bits_total(const uint32_t *bits,
@@ -1340,6 +1384,16 @@ void DeviceStitch::process(int first_rel, int nframes, bool flush_all,
         base_, pixels_out_.as<sf_pixel>());
     check(hipGetLastError(), "launching the spot pixel gather");
   }
+  // The new records back to host memory (huge-page arrays that grow without
+  // touching their pages twice), behind their kernels on this stream.
+  if (A)
+    check(hipMemcpyAsync(spots_host_.extend(A), spots_out_.as<sf_spot>() + spots_dev_,
+                         A * sizeof(sf_spot), hipMemcpyDeviceToHost, s),
+          "copying the spots back");
+  if (P)
+    check(hipMemcpyAsync(pixels_host_.extend(P), pixels_out_.as<sf_pixel>() + pixels_dev_,
+                         P * sizeof(sf_pixel), hipMemcpyDeviceToHost, s),
+          "copying the spot pixels back");
   spots_dev_ += A;
   pixels_dev_ += P;
   dead_ += sm.dead;
@@ -1383,30 +1437,14 @@ void DeviceStitch::process(int first_rel, int nframes, bool flush_all,
   }
 }
 
-const std::vector<sf_spot> &DeviceStitch::spots() {
-  if (spots_host_.size() < spots_dev_) {
-    const size_t have = spots_host_.size();
-    spots_host_.resize(spots_dev_);
-    check(hipMemcpyAsync(spots_host_.data() + have, spots_out_.as<sf_spot>() + have,
-                         (spots_dev_ - have) * sizeof(sf_spot), hipMemcpyDeviceToHost,
-                         stream_),
-          "copying the spots back");
-    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  }
-  return spots_host_;
+Records<sf_spot> DeviceStitch::spots() {
+  check(hipStreamSynchronize(stream_), "waiting for the spots' copy");
+  return {spots_host_.data(), spots_dev_};
 }
 
-const std::vector<sf_pixel> &DeviceStitch::pixels() {
-  if (pixels_host_.size() < pixels_dev_) {
-    const size_t have = pixels_host_.size();
-    pixels_host_.resize(pixels_dev_);
-    check(hipMemcpyAsync(pixels_host_.data() + have, pixels_out_.as<sf_pixel>() + have,
-                         (pixels_dev_ - have) * sizeof(sf_pixel), hipMemcpyDeviceToHost,
-                         stream_),
-          "copying the spot pixels back");
-    check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
-  }
-  return pixels_host_;
+Records<sf_pixel> DeviceStitch::pixels() {
+  check(hipStreamSynchronize(stream_), "waiting for the spot pixels' copy");
+  return {pixels_host_.data(), pixels_dev_};
 }
 
 } // namespace sfi
