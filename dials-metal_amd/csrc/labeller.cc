@@ -85,16 +85,23 @@ template <typename T> void BigArray<T>::grow(size_t need) {
   constexpr size_t kHuge = 2u << 20;
   size_t cap = std::max<size_t>({need, 2 * cap_, kHuge / sizeof(T)});
   const size_t bytes = (cap * sizeof(T) + kHuge - 1) / kHuge * kHuge;
-  cap = bytes / sizeof(T);
-  T *next = static_cast<T *>(std::aligned_alloc(kHuge, bytes));
-  if (!next)
+  void *next;
+  if (data_) // the pages move with their contents; nothing is copied
+    next = mremap(data_, bytes_, bytes, MREMAP_MAYMOVE);
+  else
+    next = mmap(nullptr, bytes, PROT_READ | PROT_WRITE, MAP_PRIVATE | MAP_ANONYMOUS,
+                -1, 0);
+  if (next == MAP_FAILED)
     throw std::bad_alloc();
   (void)madvise(next, bytes, MADV_HUGEPAGE); // a hint: small pages otherwise
-  if (size_)
-    std::memcpy(next, data_, size_ * sizeof(T));
-  std::free(data_);
-  data_ = next;
-  cap_ = cap;
+  data_ = static_cast<T *>(next);
+  bytes_ = bytes;
+  cap_ = bytes / sizeof(T);
+}
+
+template <typename T> BigArray<T>::~BigArray() {
+  if (data_)
+    (void)munmap(data_, bytes_);
 }
 
 template class BigArray<sf_pixel>;

@@ -14,15 +14,16 @@
 
 namespace sfl {
 
-// A growable array of plain records in memory advised for huge pages.  A
-// sweep's spot pixels reach hundreds of megabytes: grown in 4 KB pages (and
-// copied at every doubling) that memory cost more than the grouping itself.
+// A growable array of plain records in an anonymous mapping advised for
+// huge pages, grown by remapping (no copy).  A sweep's spot pixels reach
+// hundreds of megabytes: grown in 4 KB pages, and copied at every doubling,
+// that memory cost more than the grouping itself.
 template <typename T> class BigArray {
 public:
   BigArray() = default;
   BigArray(const BigArray &) = delete;
   BigArray &operator=(const BigArray &) = delete;
-  ~BigArray() { std::free(data_); }
+  ~BigArray();
   void push_back(const T &v) {
     if (size_ == cap_)
       grow(size_ + 1);
@@ -45,7 +46,7 @@ public:
 private:
   void grow(size_t need);
   T *data_ = nullptr;
-  size_t size_ = 0, cap_ = 0;
+  size_t size_ = 0, cap_ = 0, bytes_ = 0;
 };
 
 // dials_spots::Options (src/spots/dials_spots.hh:81-95).

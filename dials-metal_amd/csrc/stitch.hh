@@ -47,7 +47,8 @@ public:
   int64_t last_frame() const { return last_frame_; }
 
   // The records emitted so far, in host memory: each batch's are copied back
-  // while the next batch runs, and these wait for the copies.
+  // while the next batch runs, and these wait for the copies.  Valid until
+  // the next add or finish.
   Records<sf_spot> spots();
   Records<sf_pixel> pixels();
   sf_counts counts() const { return counts_; }
@@ -56,6 +57,7 @@ public:
   size_t held_bytes() const;
 
 private:
+  class Drain;
   struct Staged {
     std::unique_ptr<DeviceBuffer> buf;
     uint32_t id = 0;
@@ -90,14 +92,16 @@ private:
   DeviceBuffer membits_, memprefix_, memtotals_, mkeys_, mvals_, mkeys2_, mvals2_;
   DeviceBuffer sort_hist_, spot_members_, spot_pix_, member_pix_, live_counts_;
   DeviceBuffer frame_desc_, comp_prefix_, alive_bits_, alive_prefix_, alive_totals_;
-  DeviceBuffer spots_out_, pixels_out_, error_, scan_desc_;
+  DeviceBuffer spots_out_[2], pixels_out_[2], error_, scan_desc_;
   Pinned host_desc_, host_summary_, host_scan_;
   std::vector<Staged> staged_;
   std::vector<std::unique_ptr<DeviceBuffer>> spare_;
 
   size_t spots_dev_ = 0, pixels_dev_ = 0; // records emitted (and copies queued)
-  sfl::BigArray<sf_spot> spots_host_;
-  sfl::BigArray<sf_pixel> pixels_host_;
+  uint64_t batches_ = 0;                   // process() calls: the output set's parity
+  uint64_t out_ticket_[2] = {0, 0};        // the drain's job on each output set
+  hipEvent_t out_ready_[2] = {nullptr, nullptr};
+  std::unique_ptr<Drain> drain_;
   sf_counts counts_{};
   size_t live_ = 0;
 };

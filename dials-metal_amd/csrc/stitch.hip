@@ -41,7 +41,12 @@
 #include <cmath>
 #include <cstring>
 #include <stdexcept>
+#include <condition_variable>
+#include <deque>
+#include <exception>
+#include <mutex>
 #include <string>
+#include <thread>
 
 #include "sf_device.hh"
 
@@ -707,16 +712,16 @@ __global__ void write_summary(const uint32_t *__restrict__ ktotal,
 __global__ __launch_bounds__(256) void write_spots(
     const uint32_t *__restrict__ key_total, const uint32_t *__restrict__ decision,
     const sf_spot *__restrict__ spot_tmp, const uint64_t *__restrict__ accidx,
-    const uint64_t *__restrict__ pixoff, uint64_t spots_before,
-    uint64_t pixels_before, sf_spot *__restrict__ out,
+    const uint64_t *__restrict__ pixoff, uint64_t pixels_before,
+    sf_spot *__restrict__ out,
     uint64_t *__restrict__ spot_pix) {
   const uint32_t k = blockIdx.x * blockDim.x + threadIdx.x;
   if (k >= *key_total || decision[k] != kDecisionAccepted)
     return;
   sf_spot s = spot_tmp[k];
   s.first = pixels_before + pixoff[k];
-  out[spots_before + accidx[k]] = s;
-  spot_pix[accidx[k]] = s.first;
+  out[accidx[k]] = s; // this batch's spots only
+  spot_pix[accidx[k]] = pixoff[k];
 }
 
 __global__ __launch_bounds__(256) void list_members(
@@ -767,6 +772,7 @@ __global__ __launch_bounds__(256) void gather_members(
     ge++;
   const uint64_t at = spot_pix[s] + (npix_prefix[gs] - npix_prefix[sp]);
   const int32_t frame = static_cast<int32_t>(base + nd.frame);
+  // `out` holds this batch's pixels; `at` is relative to them.
   SF_GLOBAL uint32_t *dst = reinterpret_cast<SF_GLOBAL uint32_t *>(gmem(out) + at);
   const SF_GLOBAL uint64_t *px = gmem(nd.pix);
   for (uint32_t k = threadIdx.x & 63; k < nd.n; k += 64) {
@@ -1005,6 +1011,175 @@ static void sort_pairs(DeviceBuffer &keys, DeviceBuffer &vals, DeviceBuffer &key
   }
 }
 
+// ---- copies back to host memory --------------------------------------------
+
+namespace {
+
+// Host memory copy on several threads for large blocks: the destination's
+// pages are touched for the first time here, and faulting them in on one
+// thread is the slow part.
+void parallel_copy(void *dst, const void *src, size_t bytes) {
+  constexpr size_t kPerThread = size_t(4) << 20;
+  const size_t nt = std::min<size_t>(8, bytes / kPerThread);
+  if (nt <= 1) {
+    std::memcpy(dst, src, bytes);
+    return;
+  }
+  const size_t part = (bytes / nt + 63) & ~size_t(63);
+  std::vector<std::thread> pool;
+  for (size_t t = 1; t < nt; t++) {
+    const size_t off = t * part;
+    if (off >= bytes)
+      break;
+    pool.emplace_back([=] {
+      std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off,
+                  std::min(part, bytes - off));
+    });
+  }
+  std::memcpy(dst, src, std::min(part, bytes));
+  for (auto &t : pool)
+    t.join();
+}
+
+} // namespace
+
+// Each batch's new records go back to host memory on a thread of its own
+// while the next batch runs: the device copies them, once their kernels are
+// done, into two pinned pieces on a stream of the drain's own, and the thread
+// moves each piece on to the end of the host arrays.  The host arrays are
+// only touched on that thread until wait() says every copy is done.
+class DeviceStitch::Drain {
+public:
+  Drain() {
+    check(hipGetDevice(&device_), "hipGetDevice");
+    check(hipStreamCreateWithFlags(&copy_, hipStreamNonBlocking), "hipStreamCreate");
+    for (int i = 0; i < 2; i++) {
+      check(hipHostMalloc(&buf_[i], kPiece, hipHostMallocDefault),
+            "hipHostMalloc (the grouping's copy-back staging)");
+      check(hipEventCreateWithFlags(&piece_[i], hipEventDisableTiming), "hipEventCreate");
+    }
+    thread_ = std::thread([this] { run(); });
+  }
+  ~Drain() {
+    {
+      std::lock_guard<std::mutex> lock(m_);
+      stop_ = true;
+    }
+    cv_.notify_all();
+    thread_.join();
+    (void)hipStreamSynchronize(copy_);
+    for (int i = 0; i < 2; i++) {
+      (void)hipEventDestroy(piece_[i]);
+      (void)hipHostFree(buf_[i]);
+    }
+    (void)hipStreamDestroy(copy_);
+  }
+  Drain(const Drain &) = delete;
+  Drain &operator=(const Drain &) = delete;
+
+  // A spots and P pixels, at `spots` and `pixels` on the device, onto the
+  // host arrays after `ready` has fired; the job's ticket.
+  uint64_t submit(const sf_spot *spots, size_t A, const sf_pixel *pixels, size_t P,
+                  hipEvent_t ready) {
+    std::lock_guard<std::mutex> lock(m_);
+    jobs_.push_back(Job{spots, A, pixels, P, ready, ++submitted_});
+    cv_.notify_all();
+    return submitted_;
+  }
+  // Waits for the job `ticket` and every one before it; rethrows the first
+  // failure of a copy.
+  void wait(uint64_t ticket) {
+    std::unique_lock<std::mutex> lock(m_);
+    done_cv_.wait(lock, [&] { return done_ >= ticket; });
+    if (error_)
+      std::rethrow_exception(error_);
+  }
+  void wait_all() { wait(submitted_); }
+  const sf_spot *spots() const { return spots_.data(); }
+  const sf_pixel *pixels() const { return pixels_.data(); }
+
+private:
+  struct Job {
+    const sf_spot *spots;
+    size_t A;
+    const sf_pixel *pixels;
+    size_t P;
+    hipEvent_t ready;
+    uint64_t ticket;
+  };
+  static constexpr size_t kPiece = size_t(8) << 20;
+
+  void run() {
+    (void)hipSetDevice(device_);
+    for (;;) {
+      Job j;
+      {
+        std::unique_lock<std::mutex> lock(m_);
+        cv_.wait(lock, [&] { return stop_ || !jobs_.empty(); });
+        if (jobs_.empty())
+          return; // stopped with nothing left
+        j = jobs_.front();
+      }
+      try {
+        if (!error_) {
+          check(hipStreamWaitEvent(copy_, j.ready, 0), "hipStreamWaitEvent");
+          copy(spots_.extend(j.A), j.spots, j.A * sizeof(sf_spot));
+          copy(pixels_.extend(j.P), j.pixels, j.P * sizeof(sf_pixel));
+        }
+      } catch (...) {
+        error_ = std::current_exception();
+      }
+      {
+        std::lock_guard<std::mutex> lock(m_);
+        jobs_.pop_front();
+        done_ = j.ticket;
+      }
+      done_cv_.notify_all();
+    }
+  }
+
+  // A block through the two pieces: the next crosses PCIe while the last is
+  // copied on.
+  void copy(void *dst, const void *src, size_t bytes) {
+    char *to = static_cast<char *>(dst);
+    const char *from = static_cast<const char *>(src);
+    size_t off_[2] = {0, 0}, n_[2] = {0, 0};
+    auto finish = [&](int i) {
+      if (n_[i] == 0)
+        return;
+      check(hipEventSynchronize(piece_[i]), "waiting for a copy back");
+      parallel_copy(to + off_[i], buf_[i], n_[i]);
+      n_[i] = 0;
+    };
+    int i = 0;
+    for (size_t off = 0; off < bytes; off += kPiece, i ^= 1) {
+      finish(i);
+      const size_t n = std::min(kPiece, bytes - off);
+      check(hipMemcpyAsync(buf_[i], from + off, n, hipMemcpyDeviceToHost, copy_),
+            "copying records back");
+      check(hipEventRecord(piece_[i], copy_), "hipEventRecord");
+      off_[i] = off;
+      n_[i] = n;
+    }
+    finish(i); // the older piece first
+    finish(i ^ 1);
+  }
+
+  int device_ = 0;
+  hipStream_t copy_ = nullptr;
+  void *buf_[2] = {nullptr, nullptr};
+  hipEvent_t piece_[2] = {nullptr, nullptr};
+  sfl::BigArray<sf_spot> spots_;
+  sfl::BigArray<sf_pixel> pixels_;
+  std::mutex m_;
+  std::condition_variable cv_, done_cv_;
+  std::deque<Job> jobs_;
+  uint64_t submitted_ = 0, done_ = 0;
+  bool stop_ = false;
+  std::exception_ptr error_;
+  std::thread thread_;
+};
+
 // ---- the host side ---------------------------------------------------------
 
 DeviceStitch::DeviceStitch(size_t height, size_t width, const sfl::Options &options)
@@ -1015,10 +1190,17 @@ DeviceStitch::DeviceStitch(size_t height, size_t width, const sfl::Options &opti
   error_.ensure(8, stream_);
   launch_fill(counters_.as<uint32_t>(), sizeof(Counters) / 4, 0, stream_);
   launch_fill(error_.as<uint32_t>(), 2, 0, stream_);
+  for (hipEvent_t &e : out_ready_)
+    check(hipEventCreateWithFlags(&e, hipEventDisableTiming), "hipEventCreate");
   check(hipStreamSynchronize(stream_), "hipStreamSynchronize");
+  drain_ = std::make_unique<Drain>();
 }
 
 DeviceStitch::~DeviceStitch() {
+  drain_.reset(); // its copies read the output buffers
+  for (hipEvent_t e : out_ready_)
+    if (e)
+      (void)hipEventDestroy(e);
   if (stream_) {
     (void)hipStreamSynchronize(stream_);
     (void)hipStreamDestroy(stream_);
@@ -1347,17 +1529,19 @@ void DeviceStitch::process(int first_rel, int nframes, bool flush_all,
 
   // The spots, and their pixels, into the output lists.
   const uint64_t A = sm.accepted, P = sm.pixels, M = sm.members;
-  auto grow = [&](DeviceBuffer &b, size_t need) {
-    if (need > b.bytes)
-      b.ensure_keep(std::max(need, 2 * b.bytes), s);
-  };
-  grow(spots_out_, (spots_dev_ + A) * sizeof(sf_spot));
-  grow(pixels_out_, (pixels_dev_ + P) * sizeof(sf_pixel));
+  // This batch's records only, in the output set of its parity: the drain
+  // copies them back while the next batch runs, and the batch after that
+  // waits for the copy before it writes the set again.
+  const int par = static_cast<int>(batches_ & 1);
+  drain_->wait(out_ticket_[par]);
+  DeviceBuffer &spots_out = spots_out_[par], &pixels_out = pixels_out_[par];
+  spots_out.ensure(std::max<uint64_t>(A, 1) * sizeof(sf_spot), s);
+  pixels_out.ensure(std::max<uint64_t>(P, 1) * sizeof(sf_pixel), s);
   spot_pix_.ensure((A + 1) * 8, s);
   if (A)
     write_spots<<<blocks_for(ncand_max + 1, 256), 256, 0, s>>>(
         ktotal, decision_.as<uint32_t>(), spot_tmp_.as<sf_spot>(), accidx_.as<uint64_t>(),
-        pixoff_.as<uint64_t>(), spots_dev_, pixels_dev_, spots_out_.as<sf_spot>(),
+        pixoff_.as<uint64_t>(), pixels_dev_, spots_out.as<sf_spot>(),
         spot_pix_.as<uint64_t>());
   if (M) {
     mkeys_.ensure(M * 4, s);
@@ -1381,19 +1565,13 @@ void DeviceStitch::process(int first_rel, int nframes, bool flush_all,
     gather_members<<<blocks_for(M, 4), 256, 0, s>>>(
         nodes, mkeys_.as<uint32_t>(), mvals_.as<uint32_t>(), static_cast<uint32_t>(M),
         member_pix_.as<uint64_t>(), spot_members_.as<uint64_t>(), spot_pix_.as<uint64_t>(),
-        base_, pixels_out_.as<sf_pixel>());
+        base_, pixels_out.as<sf_pixel>());
     check(hipGetLastError(), "launching the spot pixel gather");
   }
-  // The new records back to host memory (huge-page arrays that grow without
-  // touching their pages twice), behind their kernels on this stream.
-  if (A)
-    check(hipMemcpyAsync(spots_host_.extend(A), spots_out_.as<sf_spot>() + spots_dev_,
-                         A * sizeof(sf_spot), hipMemcpyDeviceToHost, s),
-          "copying the spots back");
-  if (P)
-    check(hipMemcpyAsync(pixels_host_.extend(P), pixels_out_.as<sf_pixel>() + pixels_dev_,
-                         P * sizeof(sf_pixel), hipMemcpyDeviceToHost, s),
-          "copying the spot pixels back");
+  check(hipEventRecord(out_ready_[par], s), "hipEventRecord");
+  out_ticket_[par] = drain_->submit(spots_out.as<sf_spot>(), A, pixels_out.as<sf_pixel>(),
+                                    P, out_ready_[par]);
+  batches_++;
   spots_dev_ += A;
   pixels_dev_ += P;
   dead_ += sm.dead;
@@ -1438,13 +1616,13 @@ void DeviceStitch::process(int first_rel, int nframes, bool flush_all,
 }
 
 Records<sf_spot> DeviceStitch::spots() {
-  check(hipStreamSynchronize(stream_), "waiting for the spots' copy");
-  return {spots_host_.data(), spots_dev_};
+  drain_->wait_all();
+  return {drain_->spots(), spots_dev_};
 }
 
 Records<sf_pixel> DeviceStitch::pixels() {
-  check(hipStreamSynchronize(stream_), "waiting for the spot pixels' copy");
-  return {pixels_host_.data(), pixels_dev_};
+  drain_->wait_all();
+  return {drain_->pixels(), pixels_dev_};
 }
 
 } // namespace sfi

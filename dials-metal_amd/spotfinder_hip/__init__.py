@@ -710,21 +710,29 @@ class Labeller:
         """Device memory the staged frames still hold (device_pixels)."""
         return int(lib().sf_labeller_device_bytes(self.h))
 
-    def spots(self):
-        n = lib().sf_labeller_spot_count(self.h)
-        out = np.zeros(n, SPOT)
+    def _records(self, count, data, dtype, copy):
+        n = count(self.h)
+        if not copy:
+            # The library's own host array, read-only; valid until the next
+            # add, finish or close.
+            if not n:
+                return np.zeros(0, dtype)
+            buf = (ctypes.c_char * (n * dtype.itemsize)).from_address(data(self.h))
+            out = np.frombuffer(buf, dtype, n)
+            out.flags.writeable = False
+            return out
+        out = np.zeros(n, dtype)
         if n:
-            ctypes.memmove(_ptr(out), lib().sf_labeller_spots(self.h),
-                           n * SPOT.itemsize)
+            ctypes.memmove(_ptr(out), data(self.h), n * dtype.itemsize)
         return out
 
-    def pixels(self):
-        n = lib().sf_labeller_pixel_count(self.h)
-        out = np.zeros(n, PIXEL)
-        if n:
-            ctypes.memmove(_ptr(out), lib().sf_labeller_pixels(self.h),
-                           n * PIXEL.itemsize)
-        return out
+    def spots(self, copy=True):
+        return self._records(lib().sf_labeller_spot_count, lib().sf_labeller_spots,
+                             SPOT, copy)
+
+    def pixels(self, copy=True):
+        return self._records(lib().sf_labeller_pixel_count, lib().sf_labeller_pixels,
+                             PIXEL, copy)
 
     def counts(self):
         c = lib().sf_labeller_counts(self.h)
