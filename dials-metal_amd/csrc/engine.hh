@@ -31,6 +31,7 @@ struct FrameSlot {
   int linked_slot = -1;       // and that frame's slot
   const void *frame = nullptr;
   int kernel = 0;
+  int lane_columns = 4;       // of the fused plan it was last enqueued under
   uint32_t link_capacity = 0;
 };
 

@@ -272,6 +272,7 @@ Engine::Prepared Engine::prepare(const FrameIn *frames, const int *ids,
     FrameSlot &s = slot(ids[i]);
     s.frame = frames[i].frame;
     s.kernel = kernel;
+    s.lane_columns = p.lane_columns;
     FusedPlan plan;
     const uint32_t k = plan_fused(plan, s.height, s.width, seg, p.lane_columns);
     fp[i] = to_params(args(ids[i], frames[i].frame), plan, p.items);
@@ -397,9 +398,11 @@ void Engine::sync(const int *ids, size_t n, size_t *counts, int *status) {
   size_t first_redone = n;
   for (size_t i = 0; i < n; i++) {
     FrameSlot &s = slot(ids[i]);
-    if (host_counters_[kCounters * ids[i] + 1] != 0 && s.kernel == SF_KERNEL_FUSED) {
+    if (s.kernel == SF_KERNEL_FUSED && (host_counters_[kCounters * ids[i] + 1] != 0 ||
+                                        fused_narrow(s.width, s.lane_columns))) {
       first_redone = first_redone < i ? first_redone : i;
-      // A valid pixel beyond the fused kernel's exact range: this frame is
+      // A valid pixel beyond the fused kernel's exact range (or a frame it
+      // left alone, narrower than a lane's columns): this frame is
       // thresholded again, in its own slot, by the direct kernels, whose sums
       // are in the reference's types.
       fallbacks_++;

@@ -225,9 +225,9 @@ extern "C" int sf_threshold_noalloc(int pixel_bits, const void *dev_frame,
     sfi::check(hipMemcpyAsync(host, counter, sizeof host, hipMemcpyDeviceToHost, s),
                "reading the counters back");
     sfi::check(hipStreamSynchronize(s), "hipStreamSynchronize");
-    if (host[1] != 0) {
-      // A valid pixel beyond the fused kernel's exact range: the direct
-      // kernels, in the reference's wide sums.
+    if (host[1] != 0 || sfi::fused_narrow(width, t.lane_columns)) {
+      // A valid pixel beyond the fused kernel's exact range (or a frame too
+      // narrow for it): the direct kernels, in the reference's wide sums.
       sfi::launch_clear(fd, 1, false, 0, s, nullptr);
       sfi::launch_direct(a, reinterpret_cast<uint8_t *>(unordered),
                          reinterpret_cast<uint8_t *>(dev_out), s);

@@ -159,13 +159,17 @@ struct FusedPlan {
 };
 // Strips of a frame's width with `lane_columns` columns a lane; rows per
 // segment for a batch of `strip_rows` (the sum over its frames of height x
-// strips) under a tuning; the plan.
+// strips) under a tuning; the plan and its work items, none for a frame
+// narrower than a lane's columns (fused_narrow), which the exact kernels take.
 uint32_t strips_of(uint32_t width, int lane_columns);
 uint32_t segment_rows(uint64_t strip_rows, const Tuning &t);
 uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
                     uint32_t seg_rows, int lane_columns);
 FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,
                       uint32_t item_base);
+inline bool fused_narrow(uint32_t width, int lane_columns) {
+  return width < static_cast<uint32_t>(lane_columns);
+}
 // One persistent launch over every item of `nframes` frames (descriptors in
 // device memory, planned with `lane_columns`), mixed sizes and pixel types;
 // each frame's mask and overflow word cleared beforehand, and `queue`, the

@@ -165,39 +165,51 @@ typedef uint16_t u16x4 __attribute__((ext_vector_type(4), aligned(2)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8), aligned(2)));
 typedef uint32_t u32x4a __attribute__((ext_vector_type(4))); // 16-byte aligned
 
-// Loads this lane's KQ pixels of in-frame row y; columns outside the frame
-// read as 0 (their masked bits are constant per lane).
+// Loads KQ pixels of in-frame row y from column `cl`, where cl + KQ <= width.
+// One load instruction on every path, so the wait for a row can count the
+// loads issued since.
 template <typename T, int KQ>
-__device__ __forceinline__ void load_row(const SF_GLOBAL T *__restrict__ frame,
-                                         int width, int y, int c0,
-                                         uint32_t (&v)[KQ]) {
+__device__ __forceinline__ void load_lane(const SF_GLOBAL T *__restrict__ frame,
+                                          int width, int y, int cl,
+                                          uint32_t (&v)[KQ]) {
   const SF_GLOBAL T *row = frame + static_cast<size_t>(y) * width;
-  if (c0 >= 0 && c0 + KQ <= width) {
-    if constexpr (sizeof(T) == 4) {
+  if constexpr (sizeof(T) == 4) {
 #pragma unroll
-      for (int k = 0; k < KQ; k += 4) {
-        const u32x4 x = *reinterpret_cast<const SF_GLOBAL u32x4 *>(row + c0 + k);
-        v[k] = x[0], v[k + 1] = x[1], v[k + 2] = x[2], v[k + 3] = x[3];
-      }
-    } else if constexpr (KQ == 8) {
-      const u16x8 x = *reinterpret_cast<const SF_GLOBAL u16x8 *>(row + c0);
-#pragma unroll
-      for (int q = 0; q < 8; q++)
-        v[q] = x[q];
-    } else {
-#pragma unroll
-      for (int k = 0; k < KQ; k += 4) {
-        const u16x4 x = *reinterpret_cast<const SF_GLOBAL u16x4 *>(row + c0 + k);
-        v[k] = x[0], v[k + 1] = x[1], v[k + 2] = x[2], v[k + 3] = x[3];
-      }
+    for (int k = 0; k < KQ; k += 4) {
+      const u32x4 x = *reinterpret_cast<const SF_GLOBAL u32x4 *>(row + cl + k);
+      v[k] = x[0], v[k + 1] = x[1], v[k + 2] = x[2], v[k + 3] = x[3];
     }
-    return;
+  } else if constexpr (KQ == 8) {
+    const u16x8 x = *reinterpret_cast<const SF_GLOBAL u16x8 *>(row + cl);
+#pragma unroll
+    for (int q = 0; q < 8; q++)
+      v[q] = x[q];
+  } else {
+#pragma unroll
+    for (int k = 0; k < KQ; k += 4) {
+      const u16x4 x = *reinterpret_cast<const SF_GLOBAL u16x4 *>(row + cl + k);
+      v[k] = x[0], v[k + 1] = x[1], v[k + 2] = x[2], v[k + 3] = x[3];
+    }
   }
+}
+
+// A lane with columns outside the frame loaded from the clamped column cl =
+// c0 - sh: its column q is x[q + sh], and its outside columns read as 0.
+template <int KQ>
+__device__ __forceinline__ void shift_lane(uint32_t (&x)[KQ], int sh,
+                                           uint32_t outside) {
+  uint32_t r[KQ];
 #pragma unroll
   for (int q = 0; q < KQ; q++) {
-    const int c = c0 + q;
-    v[q] = (c >= 0 && c < width) ? static_cast<uint32_t>(row[c]) : 0u;
+    uint32_t v = 0;
+#pragma unroll
+    for (int j = 0; j < KQ; j++)
+      v = q + sh == j ? x[j] : v;
+    r[q] = (outside >> q) & 1u ? 0u : v;
   }
+#pragma unroll
+  for (int q = 0; q < KQ; q++)
+    x[q] = r[q];
 }
 
 // The high ring's form of a lane's pixels (each below 2^18): bits 16-17 of
@@ -398,9 +410,10 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
   };
   bool big = false; // a valid pixel of this lane at or above kFast
 
-  // Columns of this lane outside the frame (masked in every row), and those
-  // it outputs: inside the strip's middle and the frame.
-  uint32_t outside = 0, in_out = 0;
+  // Columns of this lane outside the frame (masked in every row), those it
+  // outputs (inside the strip's middle and the frame), and those whose 5 x 5
+  // erode reaches an output through stage 2's 11 x 11 window.
+  uint32_t outside = 0, in_out = 0, reach = 0;
 #pragma unroll
   for (int q = 0; q < KQ; q++) {
     const int c = c0 + q;
@@ -408,6 +421,8 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       outside |= 1u << q;
     else if (c >= x0 && c < x0 + kOut)
       in_out |= 1u << q;
+    if (c >= x0 - 5 && c < x0 + kOut + 5)
+      reach |= 1u << q;
   }
 
   // The ring starts at zero: rows before the first add and remove nothing.
@@ -438,22 +453,84 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
   // with a pixel at or above 2^16; rows (y_in - 5) - k with an unmasked scr1
   // pixel anywhere in the wave.
   uint32_t wide_rows = 0, x1_rows = 0;
+  // Wave-uniform: rows (y_in - 3) - k with a horizontal-erode bit anywhere in
+  // the wave, at bit k.
+  uint32_t y_rows = 0;
+  // Wave-uniform: those rows with such a bit at a pixel unmasked in its row.
+  uint32_t yu_rows = 0;
+
+  // Masks that stay put (module gaps, the frame's edges, dead columns): while
+  // every lane's masked columns are the same in rows y_in-6 .. y_in, the
+  // lattice pre-test still applies with one column per four, `sq[h]` of
+  // columns 4h .. 4h + 3: column 2 if it is unmasked, else 1, else 3.  The
+  // argument of the unmasked case carries over when every unmasked column
+  // whose erode reaches an output has such a column within two of it (a
+  // column unmasked in the row tested is unmasked in every row of the
+  // erodes through it), which the plan checks; the counts are then fixed,
+  // sm[h] valid pixels in column sq[h]'s window, with its table root.
+  uint32_t s_pat = ~0u; // the pattern planned for (per lane)
+  bool s_ok = false;    // wave-uniform: the plan covers every column
+  uint32_t s_y = 0;     // the horizontal erode of the masked columns alone
+  uint32_t sq[P], sm[P];
+  R sroot[P];
+#pragma unroll
+  for (int h = 0; h < P; h++) {
+    sq[h] = 0;
+    sm[h] = 0;
+    sroot[h] = table[0];
+  }
+  auto plan_static = [&](uint32_t pat) {
+    s_pat = pat;
+    // bit j: column c0 - 3 + j
+    const uint32_t x = ((from_left(pat) >> (KQ - 3)) & 7u) | (pat << 3) |
+                       ((from_right(pat) & 7u) << (KQ + 3));
+    uint32_t own = 0;
+#pragma unroll
+    for (int h = 0; h < P; h++) {
+      const uint32_t g = (pat >> (4 * h)) & 0xFu;
+      const uint32_t q = !(g & 4u) ? 2u : !(g & 2u) ? 1u : !(g & 8u) ? 3u : 0u;
+      const uint32_t col = 4u * h + q;
+      const uint32_t m = q ? 7u * (7u - static_cast<uint32_t>(
+                                            __popc((x >> col) & 0x7Fu)))
+                           : 0u;
+      sq[h] = q;
+      sm[h] = m;
+      sroot[h] = table[m];
+      own |= (q ? 1u : 0u) << col;
+    }
+    // bit j: column c0 - 4 + j holds a tested column
+    const uint32_t t = (from_left(own) >> (KQ - 4)) | (own << 4) |
+                       (from_right(own) << (KQ + 4));
+    const uint32_t near = t | (t >> 1) | (t >> 2) | (t >> 3) | (t >> 4);
+    const uint32_t need = ~pat & reach;
+    s_ok = __ballot(((near >> 2) & need) != need) == 0;
+    const uint32_t e = ((from_left(pat) >> (KQ - 2)) & 3u) | (pat << 2) |
+                       ((from_right(pat) & 3u) << (KQ + 2));
+    s_y = e & (e >> 1) & (e >> 2) & (e >> 3) & (e >> 4) & ((1u << KQ) - 1u);
+  };
 
   const int steps = (y1 - y0) + 20;
-  // kAhead rows in flight from HBM: the buffer a row is read from takes the
-  // load of the row kAhead below as soon as its values are in registers.
-  // (Four in flight measured no faster, alone on the device or batched.)
-  uint32_t ahead[kAhead][KQ] = {};
+  // Rows arrive kAhead ahead of use, in three buffers: the row read from one
+  // while the row kAhead below loads into the one read the step before, so
+  // no value is copied out of the way of a load.  Rows outside the frame
+  // load a clamped row (their values are replaced below), and rows past the
+  // segment's last reload that last row, from the cache.
+  const int cl = min(max(c0, 0), width - KQ);
+  const int sh = c0 - cl;
+  const bool edge = __ballot(outside != 0) != 0; // wave-uniform
+  const int y_last = min(height, y1 + 10) - 1;
+  auto issue = [&](int y, uint32_t(&v)[KQ]) {
+    load_lane<T, KQ>(frame, width, max(0, min(y, y_last)), cl, v);
+  };
+  uint32_t ahead[kAhead + 1][KQ];
 #pragma unroll
   for (int k = 0; k < kAhead; k++)
-    if (y0 - 10 + k >= 0 && y0 - 10 + k < height)
-      load_row<T, KQ>(frame, width, y0 - 10 + k, c0, ahead[k]);
+    issue(y0 - 10 + k, ahead[k]);
   int slot = 0; // ring slot of the entering row
-  auto row = [&](int t, uint32_t(&buf)[KQ]) {
+  auto row = [&](int t, uint32_t(&val)[KQ], uint32_t(&next)[KQ]) {
     const int y_in = y0 - 10 + t;
 
     // ---- the entering row: values with masked as zero, masked bits ----
-    uint32_t val[KQ];
     uint32_t mbits = outside;
     bool wide = false;
     if (y_in < 0 || y_in >= height) {
@@ -462,12 +539,12 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
         val[q] = 0;
       mbits = (1u << KQ) - 1u;
     } else {
+      if (edge)
+        shift_lane<KQ>(val, sh, outside);
       uint32_t top = 0;
 #pragma unroll
-      for (int q = 0; q < KQ; q++) {
-        val[q] = buf[q];
+      for (int q = 0; q < KQ; q++)
         top = max(top, val[q]);
-      }
       if (__builtin_expect(__ballot(top >= plain) != 0, 0)) {
         // Masked pixels become zero; a valid pixel at or above kFast voids
         // this frame's fast sums (the caller re-runs it exactly) and is
@@ -484,8 +561,7 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
         wide = __ballot(any >= kWide) != 0;
       }
     }
-    if (t + kAhead < steps && y_in + kAhead >= 0 && y_in + kAhead < height)
-      load_row<T, KQ>(frame, width, y_in + kAhead, c0, buf);
+    issue(y_in + kAhead, next);
 
     // Ring rows leaving stage 0 (y_in - 7), entering stage 2 (y_in - 5) and
     // leaving it (y_in - 16, whose slot the entering row takes once it has
@@ -552,11 +628,13 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     // ---- stage 0: 7 columns across lanes, test -> scr0 of row y_in-3 ----
     uint32_t scr0 = 0; // starts as the masked bits of row y_in-3
     uint32_t m7any = 0;
+    bool quiet = false; // wave-uniform: scr0 is a planned static mask alone
 #pragma unroll
     for (int h = 0; h < P; h++) {
       scr0 |= ((mh[h] >> 12) & 0xFu) << (4 * h);
       m7any |= mh[h] & 0x0FFFFFFFu;
     }
+    const uint32_t scr0_masked = scr0;
     // A masked pixel in rows y_in-6 .. y_in of any lane: counts vary and
     // travel in the top bits of the sums; otherwise every count is 49.
     if (__ballot(m7any != 0) == 0) {
@@ -615,49 +693,114 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
             scr0 |= 1u << q;
       }
     } else {
-      double hs[KQ];
-      across<3>(v0s, hs);
-      uint32_t word[KQ], hw[KQ];
-      // each column's unmasked pixels in rows y_in-6 .. y_in, counted
-      // directly from the clear bits of the masked history
+      // Masks that stay put take the pre-test on each lane's planned columns.
+      bool counted = true;
+      uint32_t pat = 0;
+      bool moving = false;
 #pragma unroll
-      for (int q = 0; q < KQ; q++)
-        word[q] = v0w[q] +
-                  (static_cast<uint32_t>(__popc(
-                       ~mh[q / 4] & static_cast<uint32_t>(column_bits(0, 7, q % 4))))
-                   << 26);
-      across<3>(word, hw);
+      for (int h = 0; h < P; h++) {
+        const uint32_t nib = mh[h] & 0xFu;
+        moving |= (mh[h] & 0x0FFFFFFFu) != nib * 0x1111111u;
+        pat |= nib << (4 * h);
+      }
+      if (__ballot(moving) == 0) {
+        if (__ballot(pat != s_pat) != 0)
+          plan_static(pat);
+        if (s_ok) {
+          // Column i of the row (from -3 to KQ + 2) at [i + 3].
+          double es[KQ + 6];
+          uint32_t ew[KQ + 6];
+          gather<KQ, 3>(v0s, es, std::make_integer_sequence<int, KQ + 6>{});
+          gather<KQ, 3>(v0w, ew, std::make_integer_sequence<int, KQ + 6>{});
+          bool hit = false;
 #pragma unroll
-      for (int q = 0; q < KQ; q++) {
-        const uint32_t m = hw[q] >> 26;
-        if (stage0_test<T>(m, hw[q] & (kCount0 - 1), hs[q], table[m]))
-          scr0 |= 1u << q;
+          for (int h = 0; h < P; h++) {
+            const int b = 4 * h;
+            // the windows of columns b + 2, b + 1 and b + 3
+            const double s2 = tree_sum<7>(es + b + 2);
+            const uint32_t w2 = tree_sum<7>(ew + b + 2);
+            const double s1 = (s2 - es[b + 8]) + es[b + 1];
+            const uint32_t w1 = (w2 - ew[b + 8]) + ew[b + 1];
+            const double s3 = (s2 - es[b + 2]) + es[b + 9];
+            const uint32_t w3 = (w2 - ew[b + 2]) + ew[b + 9];
+            const double ss = sq[h] == 1u ? s1 : (sq[h] == 3u ? s3 : s2);
+            const uint32_t ww = sq[h] == 1u ? w1 : (sq[h] == 3u ? w3 : w2);
+            hit |= stage0_test<T>(sm[h], ww, ss, sroot[h]);
+          }
+          // No planned column positive: scr0 is the masked pixels alone.
+          counted = __ballot(hit) != 0;
+          quiet = !counted;
+        }
+      }
+      if (counted) {
+        double hs[KQ];
+        across<3>(v0s, hs);
+        uint32_t word[KQ], hw[KQ];
+        // each column's unmasked pixels in rows y_in-6 .. y_in, counted
+        // directly from the clear bits of the masked history
+#pragma unroll
+        for (int q = 0; q < KQ; q++)
+          word[q] = v0w[q] +
+                    (static_cast<uint32_t>(__popc(
+                         ~mh[q / 4] & static_cast<uint32_t>(column_bits(0, 7, q % 4))))
+                     << 26);
+        across<3>(word, hw);
+#pragma unroll
+        for (int q = 0; q < KQ; q++) {
+          const uint32_t m = hw[q] >> 26;
+          if (stage0_test<T>(m, hw[q] & (kCount0 - 1), hs[q], table[m]))
+            scr0 |= 1u << q;
+        }
       }
     }
 
     // ---- stage 1: erode 5 x 5 -> scr1 of row y_in-5 ----
     uint32_t x1 = 0, ex = 0; // scr1 and not masked; scr1 or masked
     {
-      const uint32_t left = from_left(scr0);
-      const uint32_t right = from_right(scr0);
-      // bit i is column c0 - 2 + i
-      const uint32_t x = ((left >> (KQ - 2)) & 3u) | (scr0 << 2) |
-                         ((right & 3u) << (KQ + 2));
-      // bit i: columns c0 - 2 + i .. c0 + 2 + i all set
-      const uint32_t y = x & (x >> 1) & (x >> 2) & (x >> 3) & (x >> 4);
+      // The horizontal erode of row y_in-3 (bit i: columns c0 - 2 + i ..
+      // c0 + 2 + i all in scr0), left clear when no lane has a scr0 pixel
+      // in the row and taken from the plan when scr0 is its static mask.
+      const uint32_t m3 = scr0_masked;
+      uint32_t y = 0;
+      if (quiet) {
+        y = s_y;
+      } else if (__ballot(scr0 != 0) != 0) {
+        const uint32_t left = from_left(scr0);
+        const uint32_t right = from_right(scr0);
+        // bit i is column c0 - 2 + i
+        const uint32_t x = ((left >> (KQ - 2)) & 3u) | (scr0 << 2) |
+                           ((right & 3u) << (KQ + 2));
+        y = x & (x >> 1) & (x >> 2) & (x >> 3) & (x >> 4) & ((1u << KQ) - 1u);
+      }
+      y_rows = (y_rows << 1) | (__ballot(y != 0) != 0 ? 1u : 0u);
+      // and of them those at a pixel unmasked in the row (a positive one)
+      yu_rows = (yu_rows << 1) | (!quiet && __ballot((y & ~m3) != 0) != 0 ? 1u : 0u);
 #pragma unroll
-      for (int h = 0; h < P; h++) {
+      for (int h = 0; h < P; h++)
         hr[h] = (hr[h] << 4) | ((y >> (4 * h)) & 0xFu);
-        const uint32_t all5 = hr[h] & (hr[h] >> 4) & (hr[h] >> 8) &
-                              (hr[h] >> 12) & (hr[h] >> 16) & 0xFu;
-        // scr1 = all5 | masked; a masked pixel adds zero to every sum, so
-        // only the unmasked part of scr1 needs excluding from the values.
-        const uint32_t m5 = (mh[h] >> 20) & 0xFu;
-        x1 |= (all5 & ~m5) << (4 * h);
-        ex |= (all5 | m5) << (4 * h);
+      // scr1 = all5 | masked; a masked pixel adds zero to every sum, so
+      // only the unmasked part of scr1 needs excluding from the values.
+      // An unmasked scr1 pixel of row y_in-5 needs the horizontal erode in
+      // all five rows and, in its own row, at itself (so at a positive
+      // pixel); unless the rows have them somewhere in the wave, scr1 is
+      // just the masked pixels.
+      if ((y_rows & 0x1Fu) == 0x1Fu && ((yu_rows >> 2) & 1u)) {
+#pragma unroll
+        for (int h = 0; h < P; h++) {
+          const uint32_t all5 = hr[h] & (hr[h] >> 4) & (hr[h] >> 8) &
+                                (hr[h] >> 12) & (hr[h] >> 16) & 0xFu;
+          const uint32_t m5 = (mh[h] >> 20) & 0xFu;
+          x1 |= (all5 & ~m5) << (4 * h);
+          ex |= (all5 | m5) << (4 * h);
+        }
+        x1_rows = (x1_rows << 1) | (__ballot(x1 != 0) != 0 ? 1u : 0u);
+      } else {
+#pragma unroll
+        for (int h = 0; h < P; h++)
+          ex |= ((mh[h] >> 20) & 0xFu) << (4 * h);
+        x1_rows <<= 1;
       }
     }
-    x1_rows = (x1_rows << 1) | (__ballot(x1 != 0) != 0 ? 1u : 0u);
 
     // ---- stage 2: vertical 11-row sums of values outside scr1 ----
     {
@@ -747,11 +890,13 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     }
     slot = (slot + 1) & (kRing - 1);
   };
-  for (int t = 0; t < steps; t += kAhead) {
-#pragma unroll
-    for (int k = 0; k < kAhead; k++)
-      if (k == 0 || t + k < steps)
-        row(t + k, ahead[k]);
+  static_assert(kAhead == 2, "the buffers rotate over three steps");
+  for (int t = 0; t < steps; t += 3) {
+    row(t, ahead[0], ahead[2]);
+    if (t + 1 < steps)
+      row(t + 1, ahead[1], ahead[0]);
+    if (t + 2 < steps)
+      row(t + 2, ahead[2], ahead[1]);
   }
   flush();
   if (__ballot(big) != 0 && lane == 0)
@@ -875,6 +1020,9 @@ uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
   p.seg_rows = seg_rows;
   p.segments = (height + seg_rows - 1) / seg_rows;
   p.lane_columns = lane_columns;
+  // A frame narrower than one lane's columns is left to the exact kernels.
+  if (fused_narrow(width, lane_columns))
+    return 0;
   return p.strips * p.segments;
 }
 

@@ -205,6 +205,62 @@ def test_lane_widths_and_lattice_against_oracle(device, oracle_lib, lane_columns
         device.set_tuning(**start)
 
 
+def gapped_frame(rng, h, w, dt, widths, band=None, from_row=0):
+    """Poisson background with masked column blocks (module gaps, dead
+    columns) of the given widths every 41 columns, so that blocks start at
+    every phase of a lane's columns; bright blobs right beside each block and
+    isolated bright pixels anywhere.  `band` masks a run of rows across the
+    frame; `from_row` starts the blocks part way down."""
+    top = np.iinfo(dt).max
+    f = rng.poisson(1.5, (h, w)).astype(np.int64)
+    gaps = [(x, widths[k % len(widths)]) for k, x in enumerate(range(17, w - 14, 41))]
+    for x, g in gaps:
+        for y in rng.integers(0, h - 6, 5):
+            for xx in (x - int(rng.integers(1, 6)), x + g + int(rng.integers(0, 5))):
+                ys = slice(y, y + int(rng.integers(1, 6)))
+                xs = slice(max(0, xx - 1), min(w, xx + int(rng.integers(1, 4))))
+                f[ys, xs] += rng.integers(15, 400)
+    n = h * w // 300
+    f[rng.integers(0, h, n), rng.integers(0, w, n)] += rng.integers(20, 3000, n)
+    f = np.minimum(f, top - 2)
+    for x, g in gaps:
+        f[from_row:, x:x + g] = top
+    if band is not None:
+        f[band[0]:band[1], :] = top
+    return f.astype(dt)
+
+
+@pytest.mark.parametrize("lane_columns", [4, 8])
+def test_static_masks_against_oracle(device, oracle_lib, lane_columns):
+    """Masks that stay put down the frame (the fused kernel's pre-test on
+    one planned column per four beside them): masked blocks 1 to 13 columns
+    wide at every phase, positives right against them, the frame's edges, a
+    horizontal gap band (masks that change from row to row around it) and
+    blocks that begin part way down; bit-exact against dext."""
+    start = device.get_tuning()
+    device.set_tuning(lane_columns=lane_columns)
+    try:
+        rng = np.random.default_rng(77 + lane_columns)
+        cases = []
+        for (h, w) in [(190, 1100), (150, 533)]:
+            for dt in (np.uint16, np.uint32):
+                for widths in ([1, 2, 3], [4, 5, 6, 7], [12, 13, 1, 9]):
+                    cases.append(((h, w, dt.__name__, tuple(widths)),
+                                  gapped_frame(rng, h, w, dt, widths)))
+                cases.append(((h, w, dt.__name__, "band"),
+                              gapped_frame(rng, h, w, dt, [12, 2], band=(60, 98))))
+                cases.append(((h, w, dt.__name__, "late"),
+                              gapped_frame(rng, h, w, dt, [3, 12], from_row=47)))
+        for name, f in cases:
+            say("static masks", lane_columns, name)
+            rc, got = device.find(f)
+            _, want = oracle_lib.dext(f)
+            assert len(want) > 0, name
+            assert rc == 0 and got.tobytes() == want.tobytes(), name
+    finally:
+        device.set_tuning(**start)
+
+
 def test_stage_times(device):
     device.profile_stages(True)
     try:
