@@ -154,52 +154,86 @@ def plain_host_fed(sf, frames_host, reps=2):
                         "time, unlabelled; value with the frames in pinned memory"}
 
 
-def serving(sf, seconds, seed):
+def serving(sf, seconds, seed, scale=3.0):
     """Six concurrent sweeps -- two each of EIGER2 4M, 9M and 16M uint32 --
     emitting frames by Poisson arrival at 100-750 Hz each (rates drawn from a
-    seeded generator), replayed into the serving loop twice on the same
-    schedule: continuous batching (every frame that has arrived, across
-    sweeps, in one launch) and one frame per launch.  Threshold, ordering and
-    2D labelling linked to each sweep's previous frame, frames resident in
-    HBM (four distinct frames per sweep, cycled)."""
+    seeded generator), replayed into the serving loop on the same schedule:
+    continuous batching (every frame that has arrived, across sweeps, in one
+    launch, two launches in flight) and one frame per launch; then the same
+    sweeps at `scale` times the rates, batched; and, host-fed, with every
+    frame arriving as its bitshuffle-LZ4 chunk in host memory (decoded on the
+    device ahead of the threshold), at half and at the full rates.  Threshold, ordering and
+    2D labelling linked to each sweep's previous frame; device frames resident
+    in HBM (four distinct frames per sweep, cycled)."""
+    from spotfinder_hip import codec
     rng = np.random.default_rng(seed)
     kinds = ["4m", "4m", "9m", "9m", "16m", "16m"]
     rates = rng.uniform(100.0, 750.0, len(kinds))
-    descs, frames = [], []
+    descs, frames, chunks = [], [], []
     for i, kind in enumerate(kinds):
         d = sf.synth_desc(kind, seed=1000 + i,
                           spots_per_frame=int(rng.integers(300, 2000)), persist=2)
         descs.append(d)
-        frames.append([sf.DeviceArray.from_numpy(sf.synth_frame(d, z))
-                       for z in range(4)])
-    events = []
-    for i, r in enumerate(rates):
-        t = rng.exponential(1.0 / r)
-        while t < seconds:
-            events.append((t, i))
-            t += rng.exponential(1.0 / r)
-    events.sort()
-    taken = [0] * len(kinds)
-    ptrs = []
-    for _, i in events:
-        ptrs.append(frames[i][taken[i] % 4].ptr)
-        taken[i] += 1
-    arrivals = [t for t, _ in events]
-    out = {"sweeps": kinds, "rates_hz": [round(float(r), 1) for r in rates],
-           "seconds": seconds, "frames": len(events),
-           "target_p99_ms": 2.0}
-    for name, max_batch in (("batched", 64), ("plain_per_frame", 1)):
+        host = [sf.synth_frame(d, z) for z in range(4)]
+        frames.append([sf.DeviceArray.from_numpy(f) for f in host])
+        chunks.append([codec.bslz4_encode(f) for f in host])
+
+    def schedule(k):
+        ev_rng = np.random.default_rng(seed + 1)
+        events = []
+        for i, r in enumerate(rates * k):
+            t = ev_rng.exponential(1.0 / r)
+            while t < seconds:
+                events.append((t, i))
+                t += ev_rng.exponential(1.0 / r)
+        events.sort()
+        return events
+
+    def run(events, max_batch, host_fed):
         srv = sf.Server(max_batch=max_batch, ring=8, label=True)
         ids = [srv.open_sweep(d.height, d.width, 32) for d in descs]
-        res = srv.replay([ids[i] for _, i in events], ptrs, arrivals)
-        bad = [r for r in res if r["status"] != 0]
-        if bad:
-            raise RuntimeError(f"serving: {len(bad)} frames refused")
+        taken = [0] * len(kinds)
+
+        def replay(evs):
+            items = []
+            for _, i in evs:
+                src = chunks if host_fed else frames
+                items.append(src[i][taken[i] % 4])
+                taken[i] += 1
+            sweeps = [ids[i] for _, i in evs]
+            arrivals = [t for t, _ in evs]
+            if host_fed:
+                res = srv.replay_chunks(sweeps, items, arrivals)
+            else:
+                res = srv.replay(sweeps, [f.ptr for f in items], arrivals)
+            bad = [r for r in res if r["status"] != 0]
+            if bad:
+                raise RuntimeError(f"serving: {len(bad)} frames refused")
+
+        # Untimed: the first quarter second of the schedule, in which each
+        # sweep's slots and (host-fed) frame buffers and decoders are made.
+        replay([e for e in events if e[0] < 0.25])
+        srv.reset_stats()
+        replay(events)
         st = srv.stats()
         srv.close()
-        out[name] = {k: round(st[k], 4) for k in
-                     ("p50_ms", "p99_ms", "max_ms", "mean_ms", "frames_per_s",
-                      "mean_batch")}
+        out = {k: round(st[k], 4) for k in
+               ("p50_ms", "p99_ms", "max_ms", "mean_ms", "frames_per_s", "mean_batch")}
+        out["offered_frames_per_s"] = round(len(events) / seconds, 1)
+        return out
+
+    base, fast, half = schedule(1.0), schedule(scale), schedule(0.5)
+    raw = sum(sum(f.nbytes for f in fr) for fr in frames)
+    packed = sum(sum(len(c) for c in cs) for cs in chunks)
+    out = {"sweeps": kinds, "rates_hz": [round(float(r), 1) for r in rates],
+           "seconds": seconds, "frames": len(base), "target_p99_ms": 2.0,
+           "chunk_ratio": round(packed / raw, 4)}
+    for name, events, max_batch, host_fed in (
+            ("batched", base, 64, False), ("plain_per_frame", base, 1, False),
+            (f"batched_x{scale:g}", fast, 64, False),
+            ("host_fed_chunks_x0.5", half, 64, True),
+            ("host_fed_chunks", base, 64, True)):
+        out[name] = run(events, max_batch, host_fed)
         log(f"serving {name}: {out[name]}")
     return out
 

@@ -468,6 +468,13 @@ std::string bitshuffle_code(long code) {
 struct sf_decoder {
   hipStream_t stream = nullptr;
   bool own_stream = false;
+  // Uploads go on a stream of their own, so a load crosses PCIe while the
+  // work before it on `stream` runs: the upload waits for the last run to be
+  // done with the device staging (`consumed`), and the next run waits for
+  // the upload (`uploaded`).
+  hipStream_t upload = nullptr;
+  hipEvent_t consumed = nullptr;
+  bool ran_once = false;
   sfi::DeviceBuffer dev;
   uint8_t *host = nullptr; // pinned staging, `host_bytes` long
   size_t host_bytes = 0;
@@ -490,13 +497,17 @@ struct sf_decoder {
   std::vector<uint32_t> status;
 
   ~sf_decoder() {
+    if (upload)
+      (void)hipStreamSynchronize(upload);
     if (stream)
       (void)hipStreamSynchronize(stream);
     if (host)
       (void)hipHostFree(host);
-    for (hipEvent_t e : {uploaded, t0, t1})
+    for (hipEvent_t e : {uploaded, consumed, t0, t1})
       if (e)
         (void)hipEventDestroy(e);
+    if (upload)
+      (void)hipStreamDestroy(upload);
     if (own_stream && stream)
       (void)hipStreamDestroy(stream);
   }
@@ -504,6 +515,13 @@ struct sf_decoder {
   void load(const sf_chunk *chunks, uint32_t count);
   void run();
   int sync(int *out);
+  // The last run's status once `after` has fired, read on `read_stream`;
+  // each chunk's message (empty when it decoded) into `messages`.
+  int status_after(hipStream_t read_stream, hipEvent_t after, int *out,
+                   std::vector<std::string> *messages);
+
+private:
+  int report(int *out, std::vector<std::string> *messages);
 };
 
 namespace {
@@ -647,10 +665,13 @@ void sf_decoder::load(const sf_chunk *chunks, uint32_t count) {
   const size_t total = o_payload + payload + kPad;
 
   // The staging may still be on its way up from the last load.
-  if (uploaded)
+  if (uploaded) {
     check(hipEventSynchronize(uploaded), "waiting for the last upload");
-  else
+  } else {
     check(hipEventCreateWithFlags(&uploaded, hipEventDisableTiming), "hipEventCreate");
+    check(hipEventCreateWithFlags(&consumed, hipEventDisableTiming), "hipEventCreate");
+    check(hipStreamCreateWithFlags(&upload, hipStreamNonBlocking), "hipStreamCreate");
+  }
   dev.ensure(total, stream);
   if (total > host_bytes) {
     if (host)
@@ -670,11 +691,13 @@ void sf_decoder::load(const sf_chunk *chunks, uint32_t count) {
   kinds.assign(count, 0);
   size_t at = o_payload;
   uint32_t kz = 0;
+  std::vector<sfi::CopyPiece> copies;
+  copies.reserve(count);
   for (uint32_t i = 0; i < count; i++) {
     Planned &p = plans[i];
     kinds[i] = p.kind;
     if (p.payload_bytes)
-      std::memcpy(host + at, p.payload, p.payload_bytes);
+      copies.push_back(sfi::CopyPiece{host + at, p.payload, p.payload_bytes});
     p.dev.payload = reinterpret_cast<uint64_t>(base + at);
     cd[i] = p.dev;
     if (p.kind == kRawChunk)
@@ -690,9 +713,12 @@ void sf_decoder::load(const sf_chunk *chunks, uint32_t count) {
     at += align16(p.payload_bytes);
   }
   std::memset(host + at, 0, total - at);
-  check(hipMemcpyAsync(dev.ptr, host, total, hipMemcpyHostToDevice, stream),
+  sfi::host_copy(copies.data(), copies.size());
+  if (ran_once) // the last run's kernels are done reading the device staging
+    check(hipStreamWaitEvent(upload, consumed, 0), "hipStreamWaitEvent");
+  check(hipMemcpyAsync(dev.ptr, host, total, hipMemcpyHostToDevice, upload),
         "uploading the chunks");
-  check(hipEventRecord(uploaded, stream), "hipEventRecord");
+  check(hipEventRecord(uploaded, upload), "hipEventRecord");
   n = count;
   for (int g = 0; g < 6; g++) {
     n_items[g] = filled[g];
@@ -709,6 +735,7 @@ void sf_decoder::run() {
   uint8_t *base = static_cast<uint8_t *>(dev.ptr);
   const ChunkDev *cd = reinterpret_cast<const ChunkDev *>(base);
   uint32_t *st = reinterpret_cast<uint32_t *>(base + off_status);
+  check(hipStreamWaitEvent(stream, uploaded, 0), "hipStreamWaitEvent");
   if (timing) {
     if (!t0) {
       check(hipEventCreate(&t0), "hipEventCreate");
@@ -745,6 +772,8 @@ void sf_decoder::run() {
             "copying an uncompressed chunk");
   if (timing)
     check(hipEventRecord(t1, stream), "hipEventRecord");
+  check(hipEventRecord(consumed, stream), "hipEventRecord");
+  ran_once = true;
   timed = timing;
   ran = true;
 }
@@ -756,6 +785,25 @@ int sf_decoder::sync(int *out) {
                          n * sizeof(uint32_t), hipMemcpyDeviceToHost, stream),
           "reading the decoders' status");
   check(hipStreamSynchronize(stream), "hipStreamSynchronize");
+  return report(out, nullptr);
+}
+
+int sf_decoder::status_after(hipStream_t read_stream, hipEvent_t after, int *out,
+                             std::vector<std::string> *messages) {
+  status.assign(n, 0);
+  if (n) {
+    check(hipStreamWaitEvent(read_stream, after, 0), "hipStreamWaitEvent");
+    check(hipMemcpyAsync(status.data(), static_cast<uint8_t *>(dev.ptr) + off_status,
+                         n * sizeof(uint32_t), hipMemcpyDeviceToHost, read_stream),
+          "reading the decoders' status");
+    check(hipStreamSynchronize(read_stream), "hipStreamSynchronize");
+  }
+  return report(out, messages);
+}
+
+int sf_decoder::report(int *out, std::vector<std::string> *messages) {
+  if (messages)
+    messages->assign(n, std::string());
   std::string first;
   for (uint32_t i = 0; i < n; i++) {
     std::string msg;
@@ -782,6 +830,8 @@ int sf_decoder::sync(int *out) {
     }
     if (out)
       out[i] = msg.empty() ? 0 : -1;
+    if (messages)
+      (*messages)[i] = msg;
     if (!msg.empty() && first.empty())
       first = msg;
   }
@@ -913,3 +963,25 @@ int sf_decompress_image(const uint8_t *data, size_t size, const char *algorithm,
 }
 
 } // extern "C"
+
+namespace sfi {
+
+void decoder_load(sf_decoder *d, const sf_chunk *chunks, size_t n) {
+  d->load(chunks, static_cast<uint32_t>(n));
+}
+
+void decoder_run(sf_decoder *d) { d->run(); }
+
+int decoder_status_after(sf_decoder *d, hipStream_t read_stream, hipEvent_t after,
+                         int *status, std::vector<std::string> *messages) {
+  return d->status_after(read_stream, after, status, messages);
+}
+
+void check_chunk(const sf_chunk &c) {
+  // The frame is only named in a plan; any aligned address stands in.
+  sf_chunk probe = c;
+  probe.frame = reinterpret_cast<void *>(uintptr_t(256));
+  (void)plan_chunk(probe, 0);
+}
+
+} // namespace sfi

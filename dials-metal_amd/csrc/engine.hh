@@ -66,6 +66,22 @@ public:
   void sync(const int *ids, size_t n, size_t *counts, int *status);
   void fetch(int id, sf_signal_pixel *out, size_t n);
 
+  // Batches in flight two at a time: mark() records on the stream the end of
+  // the work enqueued so far; sync_after() waits for such a mark and reads
+  // the counters of slots ids[i] on a read stream of the engine's own, so a
+  // batch enqueued behind the mark keeps running.  A frame the fused kernel
+  // handed back is re-run as in sync() (which then waits for everything);
+  // the frames of the batch behind (`later`, linked to `later_prev`) are
+  // then labelled again, since their links may read a re-run frame.
+  void mark(hipEvent_t e);
+  void sync_after(hipEvent_t done, const int *ids, size_t n, size_t *counts,
+                  int *status, const int *later, const int *later_prev,
+                  size_t nlater);
+  // A slot's ordered list to host memory once `done` has fired, on the read
+  // stream; returns when it is there.
+  void fetch_after(hipEvent_t done, int id, sf_signal_pixel *out, size_t n);
+  hipStream_t read_stream();
+
   // Loads an ascending host list into a slot as if thresholded.
   void load_signal(int id, const sf_signal_pixel *list, size_t n,
                    uint32_t height, uint32_t width);
@@ -121,6 +137,7 @@ private:
 
   hipStream_t stream_ = nullptr;
   bool own_stream_ = false;
+  hipStream_t read_stream_ = nullptr; // made on first use
   std::vector<std::unique_ptr<FrameSlot>> slots_;
   DeviceBuffer counters_, table_, desc_;
   DeviceBuffer queue_; // the fused kernel's item queue, one word

@@ -76,8 +76,65 @@ Engine::~Engine() {
     if (s)
       (void)hipHostFree(s);
   slots_.clear();
+  if (read_stream_) {
+    (void)hipStreamSynchronize(read_stream_);
+    (void)hipStreamDestroy(read_stream_);
+  }
   if (own_stream_ && stream_)
     (void)hipStreamDestroy(stream_);
+}
+
+hipStream_t Engine::read_stream() {
+  if (!read_stream_)
+    check(hipStreamCreateWithFlags(&read_stream_, hipStreamNonBlocking),
+          "hipStreamCreate");
+  return read_stream_;
+}
+
+void Engine::mark(hipEvent_t e) { check(hipEventRecord(e, stream_), "hipEventRecord"); }
+
+void Engine::sync_after(hipEvent_t done, const int *ids, size_t n, size_t *counts,
+                        int *status, const int *later, const int *later_prev,
+                        size_t nlater) {
+  hipStream_t rs = read_stream();
+  check(hipStreamWaitEvent(rs, done, 0), "hipStreamWaitEvent");
+  check(hipMemcpyAsync(host_counters_, counters_.ptr,
+                       kCounters * sizeof(uint32_t) * slots_.size(),
+                       hipMemcpyDeviceToHost, rs),
+        "reading the counters back");
+  check(hipStreamSynchronize(rs), "hipStreamSynchronize");
+  bool redo = false;
+  for (size_t i = 0; i < n; i++) {
+    const FrameSlot &s = slot(ids[i]);
+    redo |= s.kernel == SF_KERNEL_FUSED && (host_counters_[kCounters * ids[i] + 1] != 0 ||
+                                            fused_narrow(s.width, s.lane_columns));
+  }
+  if (redo) {
+    sync(ids, n, counts, status);
+    if (nlater) {
+      label(later, later_prev, nlater);
+      read_counters();
+    }
+    return;
+  }
+  for (size_t i = 0; i < n; i++) {
+    const uint32_t c = host_counters_[kCounters * ids[i]];
+    if (counts)
+      counts[i] = c;
+    if (status)
+      status[i] = c > slot(ids[i]).capacity ? -2 : 0;
+  }
+}
+
+void Engine::fetch_after(hipEvent_t done, int id, sf_signal_pixel *out, size_t n) {
+  if (n == 0)
+    return;
+  hipStream_t rs = read_stream();
+  check(hipStreamWaitEvent(rs, done, 0), "hipStreamWaitEvent");
+  check(hipMemcpyAsync(out, slot(id).ordered.ptr, n * sizeof(sf_signal_pixel),
+                       hipMemcpyDeviceToHost, rs),
+        "copying the signal pixels back");
+  check(hipStreamSynchronize(rs), "hipStreamSynchronize");
 }
 
 void Engine::grow(DeviceBuffer &b, size_t bytes) {

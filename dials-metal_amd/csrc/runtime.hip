@@ -56,7 +56,7 @@ std::mutex g_tuning_mutex;
 bool g_tuning_read = false;
 // Measured on MI355X over 16-frame batches of 16M frames: four waves a block
 // (the most resident waves under the LDS each wave's ring takes), segments
-// of 128-256 rows; a lone frame gets 32-row segments to fill the device.
+// of 128-256 rows; a lone frame gets 40-row segments.
 sf_tuning g_tuning = {0, 8192, 4, 32, 4};
 
 bool tuning_valid(const sf_tuning &t) {

@@ -9,7 +9,9 @@
 #include <cstdint>
 #include <stdexcept>
 #include <string>
+#include <vector>
 
+#include "host_copy.hh"
 #include "spotfinder_hip.h"
 
 namespace sfi {
@@ -27,6 +29,7 @@ inline void check(hipError_t e, const char *what) {
 }
 
 void set_last_error(const std::string &message);
+
 
 // The fused kernel's launch shape (sf_tuning in the public header).
 using Tuning = sf_tuning;
@@ -252,5 +255,16 @@ void launch_set_bits(const FrameDev *frames_device, uint32_t nframes,
 // Labels every frame: components, moments, and links where prev_mask is set.
 void launch_label(const FrameDev *frames_device, uint32_t nframes,
                   hipStream_t stream);
+
+// The device decoder behind sf_decoder_*, for the serving loop: load and run
+// as sf_decoder_load/run (throwing instead of returning a status), and the
+// last run's status per chunk once `after` has fired, read on `read_stream`
+// (0 or -1, with each chunk's message).  check_chunk throws, with the
+// reference's message, for a chunk that does not describe its frame.
+void decoder_load(sf_decoder *d, const sf_chunk *chunks, size_t n);
+void decoder_run(sf_decoder *d);
+int decoder_status_after(sf_decoder *d, hipStream_t read_stream, hipEvent_t after,
+                         int *status, std::vector<std::string> *messages);
+void check_chunk(const sf_chunk &c);
 
 } // namespace sfi

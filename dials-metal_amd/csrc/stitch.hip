@@ -1015,30 +1015,12 @@ static void sort_pairs(DeviceBuffer &keys, DeviceBuffer &vals, DeviceBuffer &key
 
 namespace {
 
-// Host memory copy on several threads for large blocks: the destination's
-// pages are touched for the first time here, and faulting them in on one
-// thread is the slow part.
+// Host memory copy of a block, on the process's copy threads when large: the
+// destination's pages are touched for the first time here, and faulting them
+// in on one thread is the slow part.
 void parallel_copy(void *dst, const void *src, size_t bytes) {
-  constexpr size_t kPerThread = size_t(4) << 20;
-  const size_t nt = std::min<size_t>(8, bytes / kPerThread);
-  if (nt <= 1) {
-    std::memcpy(dst, src, bytes);
-    return;
-  }
-  const size_t part = (bytes / nt + 63) & ~size_t(63);
-  std::vector<std::thread> pool;
-  for (size_t t = 1; t < nt; t++) {
-    const size_t off = t * part;
-    if (off >= bytes)
-      break;
-    pool.emplace_back([=] {
-      std::memcpy(static_cast<char *>(dst) + off, static_cast<const char *>(src) + off,
-                  std::min(part, bytes - off));
-    });
-  }
-  std::memcpy(dst, src, std::min(part, bytes));
-  for (auto &t : pool)
-    t.join();
+  const CopyPiece piece{dst, src, bytes};
+  host_copy(&piece, 1);
 }
 
 } // namespace

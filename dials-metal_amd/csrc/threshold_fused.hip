@@ -1002,10 +1002,12 @@ uint32_t strips_of(uint32_t width, int lane_columns) {
 uint32_t segment_rows(uint64_t strip_rows, const Tuning &t) {
   // Segments tall enough that the 20 halo rows cost little, short enough
   // that the launch still spreads over the whole device: about target_waves
-  // items, each 32 to 256 rows.
+  // items, each 40 to 256 rows.  (A lone 16M frame measured fastest at about
+  // 40: shorter segments add halo rows faster than the extra waves hide the
+  // row chain's latency.)
   const uint64_t target = static_cast<uint64_t>(t.target_waves);
   uint64_t seg = (strip_rows + target - 1) / target;
-  seg = seg < 32 ? 32 : (seg > 256 ? 256 : seg);
+  seg = seg < 40 ? 40 : (seg > 256 ? 256 : seg);
   // A set length is a ceiling: a launch too small to fill the device with
   // segments that long (a lone frame, under a shape tuned for batches) takes
   // the shorter ones.

@@ -274,6 +274,9 @@ def lib():
         "sf_server_stats_get": ([vp, ctypes.POINTER(ServerStats)], ctypes.c_int),
         "sf_server_stats_reset": ([vp], None),
         "sf_server_replay": ([vp, vp, vp, vp, sz, vp], ctypes.c_int),
+        "sf_server_submit_chunk": ([vp, ctypes.c_int, vp, sz, ctypes.c_char_p, vp, sz,
+                                    ctypes.POINTER(ctypes.c_uint64)], ctypes.c_int),
+        "sf_server_replay_chunks": ([vp, vp, vp, vp, sz, vp], ctypes.c_int),
         "sf_synth_default": ([ctypes.POINTER(SynthDesc), u32, u32], None),
         "sf_synth_eiger": ([ctypes.POINTER(SynthDesc), ctypes.c_int, ctypes.c_int], None),
         "sf_synth_frame": ([ctypes.POINTER(SynthDesc), ctypes.c_int64, vp], ctypes.c_int),
@@ -771,6 +774,7 @@ class Server:
     def __init__(self, max_batch=64, ring=4, label=True, reserve_bytes=1 << 30):
         cfg = ServerConfig(max_batch, ring, int(bool(label)), reserve_bytes)
         self.h = ctypes.c_void_p()
+        self._held = {}  # chunk bytes of requests not yet waited for
         _check(lib().sf_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
 
     def close(self):
@@ -802,9 +806,25 @@ class Server:
                                       ctypes.byref(r)))
         return r.value
 
+    def submit_chunk(self, sweep, data, algorithm="bslz4", out=None):
+        """Queues a compressed chunk (bytes as HDF5 stored them) of a frame of
+        the sweep's shape; decoded on the device ahead of the threshold.  The
+        bytes are held until the request is waited for."""
+        buf = ctypes.create_string_buffer(bytes(data), len(data))
+        r = ctypes.c_uint64(0)
+        _check(lib().sf_server_submit_chunk(
+            self.h, sweep, ctypes.cast(buf, ctypes.c_void_p), len(data),
+            (algorithm or "").encode(), None if out is None else _ptr(out),
+            0 if out is None else len(out), ctypes.byref(r)))
+        self._held[r.value] = buf
+        return r.value
+
     def wait(self, request):
         res = ServerResult()
-        _check(lib().sf_server_wait(self.h, request, ctypes.byref(res)))
+        try:
+            _check(lib().sf_server_wait(self.h, request, ctypes.byref(res)))
+        finally:
+            self._held.pop(request, None)
         return res.as_dict()
 
     def stats(self):
@@ -826,6 +846,26 @@ class Server:
         _check(lib().sf_server_replay(self.h, _ptr(sw), _ptr(fr), _ptr(ar), n,
                                       ctypes.cast(out, ctypes.c_void_p)))
         return [r.as_dict() for r in out]
+
+    def replay_chunks(self, sweeps, chunks, arrivals, algorithm="bslz4"):
+        """replay() for compressed chunks: chunks[i] (bytes) for sweeps[i],
+        submitted from host memory at arrivals[i] seconds."""
+        n = len(sweeps)
+        sw = np.ascontiguousarray(sweeps, np.int32)
+        ar = np.ascontiguousarray(arrivals, np.float64)
+        held = {}
+        desc = (Chunk * max(1, n))()
+        alg = (algorithm or "").encode()
+        for i, c in enumerate(chunks):
+            if id(c) not in held:
+                held[id(c)] = ctypes.create_string_buffer(bytes(c), len(c))
+            desc[i].data = ctypes.cast(held[id(c)], ctypes.c_void_p)
+            desc[i].size = len(c)
+            desc[i].algorithm = alg
+        out = (ServerResult * max(1, n))()
+        _check(lib().sf_server_replay_chunks(self.h, _ptr(sw), ctypes.cast(desc, ctypes.c_void_p),
+                                             _ptr(ar), n, ctypes.cast(out, ctypes.c_void_p)))
+        return [out[i].as_dict() for i in range(n)]
 
 
 def workspace_label_enqueue(ws, link_previous=True):

@@ -118,7 +118,7 @@ int sf_get_kernel(void);
 /* The fused kernel's launch shape, which the autotuner writes.  A wave sweeps
  * a strip of 64 x `lane_columns` columns (4 or 8 a lane) down `seg_rows` rows
  * (plus 20 rows of halo).  Segments are sized so that a launch makes about
- * `target_waves` waves, 32 to 256 rows each; a non-zero seg_rows caps that
+ * `target_waves` waves, 40 to 256 rows each; a non-zero seg_rows caps that
  * length (so a lone frame under a shape tuned for batches still spreads over
  * the device).  `waves_per_block` is 1, 2 or 4;
  * `frames_per_launch` is the batched mode's batch.  No counterpart in the
@@ -391,11 +391,15 @@ size_t sf_labeller_live(const sf_labeller *l);
  * (find_spots.cc:732-891); this is the same per-frame work (the threshold,
  * the ordered list, 2D components linked to the sweep's previous frame)
  * behind a request loop.  A worker thread takes every frame that has
- * arrived (at most ring - 1 per sweep) into one batch -- one fused launch
- * for all of them -- while later arrivals wait for the next: continuous
- * batching.  Each sweep keeps its own ring of frame slots, so its label
+ * arrived (at most ring - 1 per sweep, counting the batch still on the
+ * device) into one batch -- one fused launch for all of them -- and enqueues
+ * it behind the batch before, whose results it then reads while the new one
+ * runs; later arrivals wait for the next: continuous batching, two batches
+ * in flight.  Each sweep keeps its own ring of frame slots, so its label
  * scratch stays resident and a frame links to its sweep's previous frame.
- * A sweep is admitted only while free HBM stays above the reserve.
+ * A sweep is admitted only while free HBM stays above the reserve.  Frames
+ * come resident in HBM (sf_server_submit) or as compressed chunks in host
+ * memory (sf_server_submit_chunk, declared after sf_chunk below).
  * ------------------------------------------------------------------------ */
 typedef struct sf_server sf_server;
 
@@ -409,7 +413,8 @@ typedef struct sf_server_config {
 typedef struct sf_server_result {
   uint64_t request;
   int32_t sweep;
-  int32_t status;          /* 0, or -2 (more signal than the list holds) */
+  int32_t status;          /* 0, -2 (more signal than the list holds), -1 (a
+                              chunk's compressed data malformed) or -3 */
   uint64_t signal_pixels;
   uint64_t components;     /* 0 unless labelling */
   uint64_t links;          /* frame links to the sweep's previous frame */
@@ -480,6 +485,25 @@ typedef struct sf_chunk {
 int sf_decompress_image(const uint8_t *data, size_t size, const char *algorithm,
                         unsigned bit_depth, uint32_t height, uint32_t width,
                         void *device_frame, size_t device_frame_bytes);
+
+/* Serving from compressed chunks: queues one chunk of a frame of the sweep's
+ * shape, in host memory exactly as HDF5 stored it, stamped with the time of
+ * the call.  The worker uploads a batch's chunks in one copy and decodes them
+ * on its stream, into frame buffers of the sweep's own (one per slot, made on
+ * its first chunk), ahead of the threshold: only the compressed bytes cross
+ * PCIe.  The bytes must stay valid until the request completes.  A chunk
+ * that does not describe its frame is refused here: -1 with the reference's
+ * message (decompress.cc:76-133); one whose compressed data are malformed
+ * completes with status -1, and sf_server_wait returns -1 with the message. */
+int sf_server_submit_chunk(sf_server *s, int sweep, const uint8_t *data,
+                           size_t size, const char *algorithm,
+                           sf_signal_pixel *host_out, size_t host_capacity,
+                           uint64_t *request);
+/* sf_server_replay for compressed chunks: chunk i (its data, size and
+ * algorithm; the other fields are the sweep's) for sweep sweeps[i]. */
+int sf_server_replay_chunks(sf_server *s, const int *sweeps, const sf_chunk *chunks,
+                            const double *arrival_s, size_t n,
+                            sf_server_result *results);
 
 /* A decoder for many chunks at once: its stream (the caller's, or NULL for
  * its own), pinned staging and device buffers, grow-only.  No counterpart in
