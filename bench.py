@@ -366,7 +366,7 @@ def decompress(sf, frames_host, distinct=4, per_step=32, steps=10):
     }
 
 
-def narrow(sf, frames_host, per_step=32, steps=10):
+def narrow(sf, frames_host, per_step=32, steps=20):
     """--gpu-force's narrowing of 32-bit frames to 16 bits with mxi_max's
     scan, on `per_step` 16M frames resident in HBM in one launch: HBM bytes 4
     read and 2 written a pixel.  The oracle's narrow_to_16 on one frame on one
@@ -381,6 +381,11 @@ def narrow(sf, frames_host, per_step=32, steps=10):
     fits, worst, f16 = O.oracle().narrow_to_16(frames_host[0])
     if res[0]["largest"] != worst or outs[0].to_numpy(np.uint16, f0.shape).tobytes() != f16.tobytes():
         raise RuntimeError("the device narrowing differs from the oracle's")
+    # Untimed launches first (about 50 ms of device work): the leg follows the
+    # lightly loaded serving replays, and a few launches measured alone can
+    # catch the device before its clocks are back up.
+    for _ in range(40):
+        sf.narrow_frames(jobs)
     ms = []
     t0 = time.perf_counter()
     for _ in range(steps):

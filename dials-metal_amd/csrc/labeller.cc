@@ -182,8 +182,6 @@ void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
                    const uint32_t *comp_of, const sf_component *comps,
                    size_t ncomp, const uint32_t *links, size_t nlinks,
                    bool links_to_previous) {
-  if (device_)
-    throw std::runtime_error("dials_spots: this labeller takes device frames");
   // Every check before any state changes, so that a refused frame leaves
   // the spots, pixels and counts as they were and the next frame groups as
   // if it had never been offered.
@@ -222,44 +220,6 @@ void Labeller::add(int64_t frame, const sf_signal_pixel *signal, size_t n,
   }
   add_nodes(frame, arena, start, n, comps, ncomp, links, nlinks,
             links_to_previous);
-}
-void Labeller::use_device_pixels() {
-  if (started_)
-    throw std::runtime_error("dials_spots: device mode is chosen before the "
-                             "first frame");
-  device_ = true;
-}
-void Labeller::add_device(int64_t frame, const void *pixels, uint64_t owner,
-                          size_t n, const sf_component *comps, size_t ncomp,
-                          const uint32_t *links, size_t nlinks,
-                          bool links_to_previous) {
-  if (!device_)
-    throw std::runtime_error("dials_spots: this labeller takes host frames");
-  if (started_ && frame <= last_frame_)
-    throw std::runtime_error("dials_spots: frame " + std::to_string(frame) +
-                             " arrived after frame " +
-                             std::to_string(last_frame_) +
-                             "; frames must be added in ascending order");
-  // The components' sizes place them in the frame's grouped pixels.
-  std::vector<uint32_t> &start = start_;
-  start.assign(ncomp + 1, 0);
-  for (size_t c = 0; c < ncomp; c++)
-    start[c + 1] = start[c] + comps[c].n;
-  if (start[ncomp] != n)
-    throw std::runtime_error("device labelling: component sizes disagree");
-  // Every check before the frame is taken: the caller releases the frames
-  // this throws for.
-  check_links(frame, ncomp, links, nlinks, links_to_previous);
-  begin_frame(frame);
-  Arena &arena = arenas_[frame];
-  arena.device = pixels;
-  arena.owner = owner;
-  add_nodes(frame, arena, start, n, comps, ncomp, links, nlinks,
-            links_to_previous);
-  // A frame without pixels is read by nothing.
-  auto it = arenas_.find(frame);
-  if (it != arenas_.end() && it->second.live == 0)
-    release(it);
 }
 void Labeller::begin_frame(int64_t frame) {
   if (started_ && frame != last_frame_ + 1)
@@ -391,13 +351,9 @@ void Labeller::flush(int64_t keep_frame) {
   // their pixels to the retained store and is released, so memory follows
   // the live pixels and not the sweep's length.  The retained store is
   // rewritten once it is mostly closed components'.
-  // (Device mode keeps each frame until its last node closes, and releases
-  // it then: nothing to do here.)
-  for (auto it = arenas_.begin();
-       !device_ && it != arenas_.end() && it->first < keep_frame;) {
+  for (auto it = arenas_.begin(); it != arenas_.end() && it->first < keep_frame;) {
     Arena &arena = it->second;
-    if (arena.live > 0 &&
-        (device_ || arena.live * kRetainRatio >= arena.index.size())) {
+    if (arena.live > 0 && arena.live * kRetainRatio >= arena.index.size()) {
       ++it;
       continue;
     }
@@ -422,12 +378,8 @@ void Labeller::flush(int64_t keep_frame) {
   }
 }
 void Labeller::release(std::map<int64_t, Arena>::iterator it) {
-  if (device_) {
-    released_.push_back(it->second.owner);
-  } else {
-    it->second.live = 0;
-    spare_arenas_.push_back(std::move(it->second));
-  }
+  it->second.live = 0;
+  spare_arenas_.push_back(std::move(it->second));
   arenas_.erase(it);
 }
 // Drops dead nodes, keeping order: roots stay each component's lowest node,
@@ -610,35 +562,6 @@ void Labeller::emit(const uint32_t *members, size_t count) {
   // The spot's pixels in (frame, index) order: its components come in
   // (frame, first) order, and those sharing a frame are merged by index.  A
   // frame with one component of the spot (nearly always) is a straight copy.
-  if (device_) {
-    // The spot's pixels are written on the device: a task per node, the
-    // nodes sharing a frame forming one group merged by index there.
-    spot.first = kept_total_;
-    uint64_t out = kept_total_;
-    size_t k = 0;
-    while (k < count) {
-      const int64_t z = nodes_[members[k]].frame;
-      size_t e = k;
-      while (e < count && nodes_[members[e]].frame == z)
-        e++;
-      const Arena &arena = arenas_.at(z);
-      const uint32_t group = static_cast<uint32_t>(tasks_.size());
-      uint64_t held = 0;
-      for (size_t j = k; j < e; j++) {
-        const Node &node = nodes_[members[j]];
-        tasks_.push_back(GatherTask{arena.device, node.begin, node.end, out,
-                                    static_cast<int32_t>(z), group,
-                                    static_cast<uint32_t>(e - k), 0});
-        held += node.end - node.begin;
-      }
-      out += held;
-      k = e;
-    }
-    kept_total_ = out;
-    spots_.push_back(spot);
-    counts_.accepted++;
-    return;
-  }
   spot.first = kept_.size();
   kept_.reserve(kept_.size() + n);
   size_t k = 0;

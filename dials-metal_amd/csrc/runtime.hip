@@ -927,7 +927,6 @@ int sf_labeller_create_device(size_t height, size_t width,
     return rc;
   return guarded([&]() -> int {
     try {
-      (*out)->lab.use_device_pixels(); // refuses host-labelled frames
       (*out)->ds = std::make_unique<DeviceStitch>(height, width, (*out)->lab.options());
       return 0;
     } catch (...) {
@@ -1028,6 +1027,8 @@ int sf_labeller_add_labelled(sf_labeller *l, int64_t frame,
       (nlinks && !links))
     return bad("labeller, list, labels, components or links");
   return add_checked(l, [&] {
+    if (l->ds) // its frames' components must be on the device
+      throw std::runtime_error("dials_spots: this labeller takes device frames");
     l->lab.add(frame, signal, n, comp_of, comps, ncomp, links, nlinks,
                links_to_previous != 0);
     l->last_engine = nullptr;

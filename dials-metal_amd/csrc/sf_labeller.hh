@@ -57,20 +57,6 @@ struct Options {
   bool two_d = false;
 };
 
-// One emitted node's pixels, for the device gather: the node's range of its
-// frame's pixels (on the device, {index, value} pairs grouped by component)
-// and where its group -- the spot's nodes on that frame, merged by index --
-// starts in the spot pixel list.
-struct GatherTask {
-  const void *pixels;
-  uint32_t begin, end;
-  uint64_t out;
-  int32_t frame;
-  uint32_t group; // the group's first task
-  uint32_t count; // tasks in the group
-  uint32_t reserved;
-};
-
 class Labeller {
 public:
   Labeller(size_t height, size_t width, const Options &options);
@@ -88,24 +74,6 @@ public:
            const uint32_t *comp_of, const sf_component *comps, size_t ncomp,
            const uint32_t *links, size_t nlinks, bool links_to_previous);
   void finish();
-
-  // Device mode, chosen before the first frame: each frame's pixels stay on
-  // the device, grouped by component ({index, value} pairs, component c at
-  // the sum of the earlier components' sizes), and the spot pixel list is
-  // written by the gather tasks this hands out.  `owner` names the device
-  // buffer holding the frame; released() lists one entry per frame no
-  // longer read, after its tasks.
-  void use_device_pixels();
-  bool device_pixels() const { return device_; }
-  void add_device(int64_t frame, const void *pixels, uint64_t owner, size_t n,
-                  const sf_component *comps, size_t ncomp,
-                  const uint32_t *links, size_t nlinks,
-                  bool links_to_previous);
-  std::vector<GatherTask> &tasks() { return tasks_; }
-  std::vector<uint64_t> &released() { return released_; }
-  // The spot pixel list's length once every task handed out has run.
-  size_t kept_total() const { return device_ ? kept_total_ : kept_.size(); }
-  BigArray<sf_pixel> &pixels_mut() { return kept_; }
 
   bool started() const { return started_; }
   int64_t last_frame() const { return last_frame_; }
@@ -128,8 +96,6 @@ private:
   };
   struct Arena {
     std::vector<uint32_t> index, value;
-    const void *device = nullptr; // device mode: the frame's pixels
-    uint64_t owner = 0;
     size_t live = 0;             // pixels of live nodes still held here
     size_t first = 0, last = 0;  // the frame's nodes, [first, last)
   };
@@ -177,10 +143,6 @@ private:
   std::vector<uint8_t> mark_;
   std::vector<size_t> pos_;
   sf_counts counts_{};
-  bool device_ = false;
-  size_t kept_total_ = 0;
-  std::vector<GatherTask> tasks_;
-  std::vector<uint64_t> released_;
 };
 
 } // namespace sfl
