@@ -242,8 +242,9 @@ def uneven_batch(sf, reps, graph):
     """One launch over an interleaved stream of 4M, 9M and 16M uint32 frames
     and 512 x 512 uint16 ROIs in the ratio 3:2:4:8, backgrounds with Poisson
     means 0.2-5 and 100-5000 spots per frame: threshold, ordering and 2D
-    labelling; frames/s and the fused kernel's achieved HBM bandwidth over
-    the batch's own bytes."""
+    labelling, under a launch shape measured for this mix (sf_autotune_mix);
+    frames/s and the fused kernel's achieved HBM bandwidth over the batch's
+    own bytes, and its time under the big batches' shape beside it."""
     rng = np.random.default_rng(17)
     kinds = ["4m"] * 3 + ["9m"] * 2 + ["16m"] * 4 + ["roi"] * 8
     rng.shuffle(kinds)
@@ -260,7 +261,16 @@ def uneven_batch(sf, reps, graph):
         f = sf.synth_frame(d, 0)
         dev.append(sf.DeviceArray.from_numpy(f))
         descs.append((dev[-1].ptr, d.height, d.width, d.bits))
+    # The mix's own launch shape, measured on these frames and kept for
+    # batches of their size (sf_autotune_mix), beside the shape the big
+    # batches' autotuning installed.
     batch = sf.Batch()
+    batch.set_timing(True)
+    batch.threshold(descs)
+    batch.sync()
+    before_ms = batch.kernel_ms()[0]
+    batch.set_timing(False)
+    tuned = sf.autotune_mix(descs)
     batch.threshold(descs, graph=graph)
     batch.label(True)
     counts, status = batch.sync()
@@ -282,6 +292,7 @@ def uneven_batch(sf, reps, graph):
     return {"frames": len(kinds), "mix": "4M:9M:16M:ROI16 = 3:2:4:8",
             "frames_per_s": len(kinds) / secs, "ms_per_batch": 1e3 * secs,
             "fused_ms": kms, "fused_GBps": byts / (kms * 1e-3) / 1e9,
+            "fused_ms_under_batch_shape": before_ms, "mix_shape": tuned["best"],
             "refused": int(sum(1 for s in status if s))}
 
 

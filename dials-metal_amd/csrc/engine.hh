@@ -117,7 +117,8 @@ private:
     size_t b_fused = 0, b_frames = 0, b_segs = 0;
     uint32_t items = 0;
     size_t max_words = 0;
-    int lane_columns = 4;
+    int lane_columns = 4, waves_per_block = 4; // the launch shape it was planned for
+    uint32_t seg_rows = 0;
   };
   void ensure_slots(size_t n);
   void grow(DeviceBuffer &b, size_t bytes);

@@ -307,7 +307,10 @@ FrameArgs Engine::args(int id, const void *frame) {
 // them) into the first half of desc_.
 Engine::Prepared Engine::prepare(const FrameIn *frames, const int *ids,
                                  size_t n, int kernel) {
-  const Tuning t = tuning();
+  uint64_t mix_rows = 0; // the batch's size, whatever shape plans it
+  for (size_t i = 0; i < n; i++)
+    mix_rows += static_cast<uint64_t>(slot(ids[i]).height) * strips_of(slot(ids[i]).width, 4);
+  const Tuning t = tuning_for_mix(mix_rows);
   Prepared p;
   p.b_fused = align16(n * sizeof(FusedParams));
   p.b_frames = align16(n * sizeof(FrameDev));
@@ -320,11 +323,13 @@ Engine::Prepared Engine::prepare(const FrameIn *frames, const int *ids,
   FrameDev *fd = reinterpret_cast<FrameDev *>(host + p.b_fused);
   ScanSeg *sg = reinterpret_cast<ScanSeg *>(host + p.b_fused + p.b_frames);
   p.lane_columns = t.lane_columns;
+  p.waves_per_block = t.waves_per_block;
   uint64_t strip_rows = 0;
   for (size_t i = 0; i < n; i++)
     strip_rows += static_cast<uint64_t>(slot(ids[i]).height) *
                   strips_of(slot(ids[i]).width, p.lane_columns);
   const uint32_t seg = segment_rows(strip_rows, t);
+  p.seg_rows = seg;
   for (size_t i = 0; i < n; i++) {
     FrameSlot &s = slot(ids[i]);
     s.frame = frames[i].frame;
@@ -363,7 +368,7 @@ void Engine::enqueue_kernels(const Prepared &p, const FrameIn *frames,
     launch_fused_batch(fp_d, nf, p.items, table_.as<float>(),
                        reinterpret_cast<const double *>(table_.as<char>() +
                                                         128 * sizeof(float)),
-                       queue_.as<uint32_t>(), p.lane_columns, stream_);
+                       queue_.as<uint32_t>(), p.lane_columns, p.waves_per_block, stream_);
   } else {
     for (size_t i = 0; i < n; i++)
       launch_direct(args(ids[i], frames[i].frame), scr0_.as<uint8_t>(),
@@ -413,17 +418,15 @@ void Engine::threshold(const FrameIn *frames, const int *ids, size_t n,
   // just before each launch, outside the graph, and the engine buffers it
   // names change the epoch when they move; so one replay serves any batch of
   // these shapes, in either set of slots.
-  const sf_tuning t = tuning();
+  const Prepared p = prepare(frames, ids, n, kernel);
   std::vector<uint64_t> key = {static_cast<uint64_t>(kernel), epoch_,
-                               static_cast<uint64_t>(t.seg_rows),
-                               static_cast<uint64_t>(t.target_waves),
-                               static_cast<uint64_t>(t.waves_per_block),
-                               static_cast<uint64_t>(t.lane_columns)};
+                               static_cast<uint64_t>(p.seg_rows),
+                               static_cast<uint64_t>(p.waves_per_block),
+                               static_cast<uint64_t>(p.lane_columns)};
   for (size_t i = 0; i < n; i++)
     key.push_back((static_cast<uint64_t>(frames[i].height) << 32) |
                   (static_cast<uint64_t>(frames[i].width) << 8) |
                   static_cast<uint64_t>(frames[i].bits));
-  const Prepared p = prepare(frames, ids, n, kernel);
   if (!(graph_exec_ && key == graph_key_)) {
     if (graph_exec_)
       check(hipGraphExecDestroy(graph_exec_), "hipGraphExecDestroy");

@@ -218,7 +218,7 @@ extern "C" int sf_threshold_noalloc(int pixel_bits, const void *dev_frame,
     const sfi::ScanSeg *sg = &desc_d->seg;
     sfi::launch_clear(fd, 1, true, 0, s, queue);
     sfi::launch_fused_batch(&desc_d->fused, 1, items, tf_d, td_d, queue,
-                            t.lane_columns, s);
+                            t.lane_columns, t.waves_per_block, s);
     sfi::launch_popcount_scan(sg, 1, L.words, s, true);
     sfi::launch_scatter(fd, 1, s);
     uint32_t host[2] = {0, 0};

@@ -15,6 +15,7 @@
 #include <cstring>
 #include <functional>
 #include <memory>
+#include <map>
 #include <mutex>
 #include <string>
 #include <vector>
@@ -54,6 +55,21 @@ void resolve_kernel() {
 
 std::mutex g_tuning_mutex;
 bool g_tuning_read = false;
+// Launch shapes measured for batches of one size: by the batch's strip-rows,
+// two buckets an octave (sf_autotune_mix).
+std::map<int, sf_tuning> g_mix_shapes;
+// Raised while an autotuner measures shapes: every batch then launches with
+// the shape being measured, whatever sf_autotune_mix kept.
+std::atomic<int> g_mix_bypass{0};
+struct MixBypass {
+  MixBypass() { g_mix_bypass++; }
+  ~MixBypass() { g_mix_bypass--; }
+};
+
+int mix_bucket(uint64_t mix_rows) {
+  return static_cast<int>(std::lround(2.0 * std::log2(static_cast<double>(
+                                                std::max<uint64_t>(mix_rows, 1)))));
+}
 // Measured on MI355X over 16-frame batches of 16M frames: four waves a block
 // (the most resident waves under the LDS each wave's ring takes), segments
 // of 128-256 rows; a lone frame gets 40-row segments.
@@ -120,6 +136,19 @@ Tuning tuning() {
     read_tuning_env();
   }
   return g_tuning;
+}
+
+Tuning tuning_for_mix(uint64_t mix_rows) {
+  const Tuning global = tuning();
+  if (g_mix_bypass.load() != 0)
+    return global;
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  const auto it = g_mix_shapes.find(mix_bucket(mix_rows));
+  if (it == g_mix_shapes.end())
+    return global;
+  Tuning t = it->second;
+  t.frames_per_launch = global.frames_per_launch; // a batched-mode setting
+  return t;
 }
 
 } // namespace sfi
@@ -785,6 +814,7 @@ int sf_autotune(const sf_frame_desc *frames, size_t n, int label,
     return bad("frames or report");
   *report = sf_autotune_report{};
   const sf_tuning start = tuning();
+  const MixBypass measuring; // the shapes tried, not a kept one, launch
   sf_batch *b = nullptr;
   int rc = sf_batch_create(nullptr, &b);
   if (rc != 0)
@@ -886,6 +916,84 @@ int sf_autotune(const sf_frame_desc *frames, size_t n, int label,
   if ((rc = sf_set_tuning(&best)) != 0)
     return fail(rc);
   return 0;
+}
+
+namespace {
+uint64_t mix_rows_of(const sf_frame_desc *frames, size_t n) {
+  uint64_t rows = 0;
+  for (size_t i = 0; i < n; i++)
+    rows += static_cast<uint64_t>(frames[i].height) * strips_of(frames[i].width, 4);
+  return rows;
+}
+} // namespace
+
+int sf_autotune_mix(const sf_frame_desc *frames, size_t n, sf_autotune_report *report) {
+  if (frames == nullptr || n == 0 || n > 4096 || report == nullptr)
+    return bad("frames or report");
+  *report = sf_autotune_report{};
+  const sf_tuning start = tuning();
+  const int bucket = mix_bucket(mix_rows_of(frames, n));
+  const MixBypass measuring;
+  auto restore = [&](int code) {
+    (void)sf_set_tuning(&start);
+    return code;
+  };
+  sf_batch *b = nullptr;
+  int rc = sf_batch_create(nullptr, &b);
+  if (rc != 0)
+    return restore(rc);
+  std::unique_ptr<sf_batch, void (*)(sf_batch *)> own(b, sf_batch_destroy);
+  std::vector<size_t> counts(n);
+  std::vector<int> status(n);
+  sf_batch_set_timing(b, 1);
+  sf_tuning best = start;
+  double best_ms = 0.0;
+  for (int kq : {4, 8})
+    for (int w : {1, 2, 4})
+      for (int k = 0; k < 7; k++) {
+        const int seg_rows[] = {64, 128, 192, 256, 0, 0, 0};
+        const int target[] = {start.target_waves, start.target_waves,
+                              start.target_waves, start.target_waves, 4096, 8192, 16384};
+        sf_tuning t = start;
+        t.lane_columns = kq;
+        t.waves_per_block = w;
+        t.seg_rows = seg_rows[k];
+        t.target_waves = target[k];
+        if ((rc = sf_set_tuning(&t)) != 0)
+          return restore(rc);
+        double ms[3];
+        for (double &m : ms)
+          if ((rc = sf_batch_threshold(b, frames, n, 0)) != 0 ||
+              (rc = sf_batch_sync(b, counts.data(), status.data())) != 0 ||
+              (rc = sf_batch_kernel_ms(b, &m, nullptr)) != 0)
+            return restore(rc);
+        std::sort(ms, ms + 3);
+        report->shapes_tried++;
+        if (best_ms == 0.0 || ms[1] < best_ms) {
+          best_ms = ms[1];
+          best = t;
+        }
+      }
+  (void)sf_set_tuning(&start);
+  report->best = best;
+  report->fused_ms_per_frame = best_ms / static_cast<double>(n);
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  g_mix_shapes[bucket] = best;
+  return 0;
+}
+
+int sf_tuning_for_mix(const sf_frame_desc *frames, size_t n, sf_tuning *t) {
+  if (frames == nullptr || n == 0 || t == nullptr)
+    return bad("frames or tuning");
+  const uint64_t rows = mix_rows_of(frames, n);
+  *t = tuning_for_mix(rows);
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  return g_mix_shapes.count(mix_bucket(rows)) ? 1 : 0;
+}
+
+void sf_clear_mix_tunings(void) {
+  std::lock_guard<std::mutex> lock(g_tuning_mutex);
+  g_mix_shapes.clear();
 }
 
 // ---- grouping --------------------------------------------------------------

@@ -34,6 +34,11 @@ void set_last_error(const std::string &message);
 // The fused kernel's launch shape (sf_tuning in the public header).
 using Tuning = sf_tuning;
 Tuning tuning();
+// The launch shape for a batch whose frames make `mix_rows` strip-rows (the
+// sum over its frames of height x strips at four columns a lane): the shape
+// sf_autotune_mix measured for batches of that size, where there is one,
+// else tuning().
+Tuning tuning_for_mix(uint64_t mix_rows);
 
 // A grow-only device allocation.  Growing waits for the stream first: work in
 // flight may still be reading the old buffer (dext_cuda.cu:506-525).
@@ -179,7 +184,8 @@ inline bool fused_narrow(uint32_t width, int lane_columns) {
 // device word from which waves take items, zeroed.
 void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
                         uint32_t total_items, const float *tf, const double *td,
-                        uint32_t *queue, int lane_columns, hipStream_t stream);
+                        uint32_t *queue, int lane_columns, int waves_per_block,
+                        hipStream_t stream);
 
 // Clears `n` 32-bit words with a kernel, which unlike a memset node stays
 // correct when the sequence is captured into a graph and replayed.

@@ -1030,10 +1030,10 @@ uint32_t plan_fused(FusedPlan &p, uint32_t height, uint32_t width,
 
 void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
                         uint32_t total_items, const float *tf, const double *td,
-                        uint32_t *queue, int lane_columns, hipStream_t stream) {
+                        uint32_t *queue, int lane_columns, int waves_per_block,
+                        hipStream_t stream) {
   if (total_items == 0)
     return;
-  const Tuning t = tuning();
   int cus = 256;
   int device = 0;
   if (hipGetDevice(&device) == hipSuccess)
@@ -1041,11 +1041,11 @@ void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
                                 device);
   const uint32_t n = static_cast<uint32_t>(cus);
   if (lane_columns == 8)
-    launch_shape<8>(t.waves_per_block, frames_device, nframes, total_items, tf,
-                    td, queue, n, stream);
+    launch_shape<8>(waves_per_block, frames_device, nframes, total_items, tf, td,
+                    queue, n, stream);
   else
-    launch_shape<4>(t.waves_per_block, frames_device, nframes, total_items, tf,
-                    td, queue, n, stream);
+    launch_shape<4>(waves_per_block, frames_device, nframes, total_items, tf, td,
+                    queue, n, stream);
   check(hipGetLastError(), "launching the fused threshold");
 }
 

@@ -261,6 +261,9 @@ def lib():
         "sf_labeller_live": ([vp], sz),
         "sf_autotune": ([vp, sz, ctypes.c_int, ctypes.POINTER(AutotuneReport)],
                         ctypes.c_int),
+        "sf_autotune_mix": ([vp, sz, ctypes.POINTER(AutotuneReport)], ctypes.c_int),
+        "sf_tuning_for_mix": ([vp, sz, ctypes.POINTER(Tuning)], ctypes.c_int),
+        "sf_clear_mix_tunings": ([], None),
         "sf_server_config_default": ([ctypes.POINTER(ServerConfig)], None),
         "sf_server_create": ([ctypes.POINTER(ServerConfig), ctypes.POINTER(vp)],
                              ctypes.c_int),
@@ -761,6 +764,33 @@ def autotune(frames, label=True):
     _check(lib().sf_autotune(ctypes.cast(arr, ctypes.c_void_p), len(frames),
                              int(bool(label)), ctypes.byref(rep)))
     return rep.as_dict()
+
+
+def autotune_mix(frames):
+    """sf_autotune_mix over device frames given as (ptr, height, width, bits):
+    the fused kernel's shape measured on this batch and kept for batches of
+    its size; the process-wide tuning is left alone.  The report as a dict."""
+    arr = (FrameDesc * len(frames))(*[FrameDesc(p, h, w, b) for p, h, w, b in frames])
+    rep = AutotuneReport()
+    _check(lib().sf_autotune_mix(ctypes.cast(arr, ctypes.c_void_p), len(frames),
+                                 ctypes.byref(rep)))
+    return rep.as_dict()
+
+
+def tuning_for_mix(frames):
+    """The tuning a batch of these frames launches with, and whether it is a
+    shape sf_autotune_mix kept for its size."""
+    arr = (FrameDesc * len(frames))(*[FrameDesc(p, h, w, b) for p, h, w, b in frames])
+    t = Tuning()
+    rc = lib().sf_tuning_for_mix(ctypes.cast(arr, ctypes.c_void_p), len(frames),
+                                 ctypes.byref(t))
+    if rc < 0:
+        _check(rc)
+    return t.as_dict(), bool(rc)
+
+
+def clear_mix_tunings():
+    lib().sf_clear_mix_tunings()
 
 
 class NotAdmitted(SpotfinderError):

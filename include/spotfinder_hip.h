@@ -320,6 +320,18 @@ typedef struct sf_autotune_report {
 } sf_autotune_report;
 int sf_autotune(const sf_frame_desc *frames, size_t n, int label,
                 sf_autotune_report *report);
+/* The fused kernel's shape for batches like this one: the same 42 shapes
+ * measured on these frames (all of them, at most 4096), the winner kept for
+ * every later batch of about this size -- its frames' height x strips at four
+ * columns a lane, within a factor of the square root of two -- leaving the
+ * process-wide tuning (and frames per launch) as they were.  `best` and
+ * `fused_ms_per_frame` report it.  No counterpart in the reference. */
+int sf_autotune_mix(const sf_frame_desc *frames, size_t n, sf_autotune_report *report);
+/* The tuning a batch of these frames would launch with; 1 when it comes from
+ * sf_autotune_mix, 0 when it is the process-wide one. */
+int sf_tuning_for_mix(const sf_frame_desc *frames, size_t n, sf_tuning *t);
+/* Forgets every shape sf_autotune_mix kept. */
+void sf_clear_mix_tunings(void);
 
 /* ------------------------------------------------------------------------
  * Grouping into spots: dials_spots::Labeller (src/spots/dials_spots.hh:

@@ -183,6 +183,52 @@ def test_autotune_installs_a_shape_and_results_do_not_change(device):
         device.set_tuning(**start)
 
 
+def test_mix_shape_is_kept_for_its_size_only(device):
+    """sf_autotune_mix measures a shape on a mixed batch and keeps it for
+    batches of that size: the process-wide tuning is unchanged, a batch of
+    the same frames reports (and launches with) the kept shape, a batch of a
+    very different size does not, every frame's list is unchanged under it,
+    and clearing forgets it."""
+    mix = [device.synth_desc("4m", spots_per_frame=400, persist=1, seed=61),
+           device.synth_desc((512, 512), bits=16, seed=62, spots_per_frame=30,
+                             ring_radius=150.0),
+           device.synth_desc("9m", spots_per_frame=800, persist=1, seed=63)]
+    host = [device.synth_frame(d, 0) for d in mix] * 2
+    descs_all = [d for d in mix] * 2
+    dev = [device.DeviceArray.from_numpy(f) for f in host]
+    descs = [(a.ptr, d.height, d.width, d.bits) for a, d in zip(dev, descs_all)]
+    start = device.get_tuning()
+    device.clear_mix_tunings()
+    batch = device.Batch()
+    batch.threshold(descs)
+    counts, status = batch.sync()
+    before = [batch.fetch(i, counts[i]) for i in range(len(descs))]
+    try:
+        report = device.autotune_mix(descs)
+        say("mix shape", report)
+        assert report["shapes_tried"] == 42 and report["fused_ms_per_frame"] > 0
+        assert device.get_tuning() == start
+        shape, kept = device.tuning_for_mix(descs)
+        assert kept and shape["seg_rows"] == report["best"]["seg_rows"]
+        for k in ("target_waves", "waves_per_block", "lane_columns"):
+            assert shape[k] == report["best"][k], k
+        assert shape["frames_per_launch"] == start["frames_per_launch"]
+        _, kept_small = device.tuning_for_mix(descs[1:2])
+        assert not kept_small
+        for graph in (False, True):
+            batch.threshold(descs, graph=graph)
+            counts2, status2 = batch.sync()
+            assert list(counts2) == list(counts) and list(status2) == list(status)
+            for i in range(len(descs)):
+                assert batch.fetch(i, counts2[i]).tobytes() == before[i].tobytes(), (graph, i)
+        device.clear_mix_tunings()
+        assert device.tuning_for_mix(descs) == (start, False)
+    finally:
+        device.clear_mix_tunings()
+        device.set_tuning(**start)
+        batch.close()
+
+
 def test_batch_size_ramp_keeps_slots_bounded(device):
     """Batches of 1, 2, ..., 8 frames in turn: each links its first frame to
     the previous batch's last, the grouping across all of them equals the
