@@ -166,3 +166,17 @@ def test_policy_environment_setters_and_announcements():
             "HIP: the three stages fused, one kernel",
             "HIP: stage0 direct window, stage2 direct window",
             "HIP: the three stages fused, one kernel"], env
+
+
+def test_host_copy_pool_under_thread_sanitizer():
+    """The host copy pool that decoders and the device grouping's drain share,
+    driven from four threads at once, built with ThreadSanitizer and run
+    (`make -C dials-metal_amd check-tsan`): every copy exact, no race."""
+    import subprocess
+    import pytest
+    if not os.path.exists("/opt/rocm/llvm/bin/clang++"):
+        pytest.skip("no clang++ with ThreadSanitizer's runtime")
+    r = subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "dials-metal_amd"),
+                        "check-tsan"], capture_output=True, text=True, timeout=600)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-4000:]
+    assert "four threads: ok" in r.stdout
