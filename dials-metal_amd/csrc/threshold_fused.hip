@@ -75,6 +75,14 @@ constexpr uint32_t kWide = 1u << 16;    // pixels at or above this use the high 
 constexpr int kTable = 64;              // root table entries used (m <= 49)
 constexpr uint32_t kCount0 = 1u << 26;  // count unit in stage 0's word
 constexpr uint32_t kCount2 = 1u << 25;  // count unit in stage 2's word
+constexpr uint32_t kHot = 1u << 13;     // uint32 pixels from here split their squares
+constexpr uint32_t kSplit = 1u << 26;   // where a square splits into low and high parts
+// Strips start kPhase columns left of a multiple of their output width, so
+// that every lane's first column is a multiple of four: a lane is then wholly
+// inside or outside a frame whose width is a multiple of four, and the
+// lattice columns 4k + 2 of the frame lie two columns before a module gap
+// that starts on a multiple of four and two after one that ends before one.
+constexpr int kPhase = 2;
 
 // Output columns of one strip with KQ columns a lane.
 __host__ __device__ constexpr int out_columns(int kq) {
@@ -96,21 +104,6 @@ __device__ __forceinline__ uint32_t from_right(uint32_t v) {
   return static_cast<uint32_t>(
       __builtin_amdgcn_mov_dpp(static_cast<int>(v), 0x130, 0xf, 0xf, true));
 }
-__device__ __forceinline__ double from_left(double v) {
-  const uint64_t b = __double_as_longlong(v);
-  const uint32_t lo = from_left(static_cast<uint32_t>(b));
-  const uint32_t hi = from_left(static_cast<uint32_t>(b >> 32));
-  return __longlong_as_double(static_cast<long long>(
-      (static_cast<uint64_t>(hi) << 32) | lo));
-}
-__device__ __forceinline__ double from_right(double v) {
-  const uint64_t b = __double_as_longlong(v);
-  const uint32_t lo = from_right(static_cast<uint32_t>(b));
-  const uint32_t hi = from_right(static_cast<uint32_t>(b >> 32));
-  return __longlong_as_double(static_cast<long long>(
-      (static_cast<uint64_t>(hi) << 32) | lo));
-}
-
 // The value that lane (this lane + D) holds.
 template <int D, typename V> __device__ __forceinline__ V from_lane(V v) {
   if constexpr (D < 0)
@@ -126,35 +119,77 @@ template <int KQ, int I> constexpr int lane_of() {
   return I >= 0 ? I / KQ : -((-I + KQ - 1) / KQ);
 }
 
-// e[j] = the value of column c0 - R + j, for j < KQ + 2R.
-template <int KQ, int R, typename V, int... J>
-__device__ __forceinline__ void gather(const V (&v)[KQ], V (&e)[KQ + 2 * R],
-                                       std::integer_sequence<int, J...>) {
-  ((e[J] = from_lane<lane_of<KQ, J - R>()>(
-        v[(J - R) - lane_of<KQ, J - R>() * KQ])),
-   ...);
-}
-
-template <int N, typename V> __device__ __forceinline__ V tree_sum(const V *e) {
-  if constexpr (N == 1)
-    return e[0];
+// Sum of columns [A, B] of v, numbered from this lane's first column: one
+// partial sum per lane that holds some of them, moved across by DPP (the
+// move folds into the add that takes it).  Partial sums run from each lane's
+// last column down, so that the sums shared between windows are formed once.
+template <int KQ, int Lo, int Hi>
+__device__ __forceinline__ uint32_t run_sum(const uint32_t (&v)[KQ]) {
+  if constexpr (Lo == Hi)
+    return v[Lo];
   else
-    return tree_sum<N / 2>(e) + tree_sum<N - N / 2>(e + N / 2);
+    return v[Lo] + run_sum<KQ, Lo + 1, Hi>(v);
+}
+template <int KQ, int A, int B, int D = lane_of<KQ, A>()>
+__device__ __forceinline__ uint32_t window_sum(const uint32_t (&v)[KQ]) {
+  constexpr int first = D * KQ;
+  constexpr int lo = (A > first ? A : first) - first;
+  constexpr int hi = (B < first + KQ - 1 ? B : first + KQ - 1) - first;
+  const uint32_t part = from_lane<D>(run_sum<KQ, lo, hi>(v));
+  if constexpr (first + KQ - 1 >= B)
+    return part;
+  else
+    return window_sum<KQ, A, B, D + 1>(v) + part;
 }
 
-// Sums over the 2R + 1 columns centred on each of a lane's KQ, with the
-// neighbouring lanes' columns by lane shifts: the first as a tree, each next
-// one from the last by the column leaving and the one entering.  The values
-// are integers (exact in doubles below 2^53, or modular in 32 bits), so the
+// The value of column I (numbered as above).
+template <int KQ, int I>
+__device__ __forceinline__ uint32_t column(const uint32_t (&v)[KQ]) {
+  return from_lane<lane_of<KQ, I>()>(v[I - lane_of<KQ, I>() * KQ]);
+}
+
+// One window's sum from its neighbour's: o - out + in for the columns Out
+// leaving and In entering, formed as in - (out - o), so that each lane shift
+// is the minuend of a subtraction and folds into it as v_sub_u32 with DPP.
+// The other order folds into v_subrev_u32 with DPP, which on this device
+// took the wrong operand in a probe of exactly this code; the empty asm
+// keeps the compiler from reassociating back into it.
+template <int KQ, int Out, int In>
+__device__ __forceinline__ uint32_t slide(uint32_t o, const uint32_t (&v)[KQ]) {
+  uint32_t t = column<KQ, Out>(v) - o;
+  __asm__("" : "+v"(t));
+  return column<KQ, In>(v) - t;
+}
+
+// Window sums of radius R at every column of the lane, given those at the
+// lattice columns 4h + 2: each next one from its neighbour by the column
+// leaving and the one entering.  The values are integers modulo 2^32, so the
 // order of the additions does not change the sums.
-template <int R, int KQ, typename V>
-__device__ __forceinline__ void across(const V (&v)[KQ], V (&h)[KQ]) {
-  V e[KQ + 2 * R];
-  gather<KQ, R>(v, e, std::make_integer_sequence<int, KQ + 2 * R>{});
-  h[0] = tree_sum<2 * R + 1>(e);
-#pragma unroll
-  for (int q = 1; q < KQ; q++)
-    h[q] = (h[q - 1] - e[q - 1]) + e[q + 2 * R];
+template <int R, int KQ, int G>
+__device__ __forceinline__ void slide_group(const uint32_t (&v)[KQ],
+                                            uint32_t (&o)[KQ]) {
+  constexpr int c = 4 * G + 2;
+  o[c - 1] = slide<KQ, c + R, c - 1 - R>(o[c], v);
+  o[c - 2] = slide<KQ, c - 1 + R, c - 2 - R>(o[c - 1], v);
+  o[c + 1] = slide<KQ, c - R, c + 1 + R>(o[c], v);
+}
+template <int R, int KQ, int... G>
+__device__ __forceinline__ void slide_all(const uint32_t (&v)[KQ],
+                                          uint32_t (&o)[KQ],
+                                          std::integer_sequence<int, G...>) {
+  (slide_group<R, KQ, G>(v, o), ...);
+}
+template <int R, int KQ, int... G>
+__device__ __forceinline__ void lattice_sums(const uint32_t (&v)[KQ],
+                                             uint32_t (&o)[KQ],
+                                             std::integer_sequence<int, G...>) {
+  ((o[4 * G + 2] = window_sum<KQ, 4 * G + 2 - R, 4 * G + 2 + R>(v)), ...);
+}
+template <int R, int KQ>
+__device__ __forceinline__ void row_sums(const uint32_t (&v)[KQ],
+                                         uint32_t (&o)[KQ]) {
+  lattice_sums<R>(v, o, std::make_integer_sequence<int, KQ / 4>{});
+  slide_all<R>(v, o, std::make_integer_sequence<int, KQ / 4>{});
 }
 
 // A lane's KQ pixels as vector loads; dword alignment is all the hardware
@@ -229,44 +264,54 @@ __device__ __forceinline__ void add_high(uint32_t h, uint32_t (&v)[KQ]) {
     v[q] |= ((h >> (2 * q)) & 3u) << 16;
 }
 
-// Stage 0's test (dext.cc:99-110) on exact window sums: `count` valid pixels
-// summing to `sum`, squares summing to `sum_sq`; the table holds
-// R(sqrt(2 (m - 1))), +inf below two, so that m < 2 tests false.
-template <typename T>
-__device__ __forceinline__ bool stage0_test(uint32_t count, uint32_t sum,
-                                            double sum_sq,
-                                            typename traits<T>::R root);
-
-template <>
-__device__ __forceinline__ bool stage0_test<uint32_t>(uint32_t count,
-                                                      uint32_t sum,
-                                                      double sum_sq,
-                                                      double root) {
-  const double m = static_cast<double>(count);
-  const double s = static_cast<double>(sum);
-  // (m * s2 - s * s) - s * (m - 1): every term and partial result is an
-  // integer below 2^53, so this one-rounding form is the same exact value.
-  const double lhs = __fma_rn(m, sum_sq, -(s * (s + (m - 1.0))));
-  const double rhs = (s * 6.0) * root;
-  return lhs > rhs;
+// 1 when `c` holds in any lane, 0 otherwise: a wave-uniform word formed in
+// scalar registers (the ballot's population count, at most one).
+__device__ __forceinline__ uint32_t bit_any(bool c) {
+  return min(static_cast<uint32_t>(__popcll(__ballot(c))), 1u);
 }
 
-template <>
-__device__ __forceinline__ bool stage0_test<uint16_t>(uint32_t count,
-                                                      uint32_t sum,
-                                                      double sum_sq,
-                                                      float root) {
-  const float m = static_cast<float>(count);
-  const float s = static_cast<float>(sum);
-  // The reference's accumulator is uint32 and wraps; the exact sum here is
-  // below 2^38, so its low 32 bits are the wrapped value.
-  const uint32_t wrapped =
-      static_cast<uint32_t>(static_cast<uint64_t>(sum_sq));
-  const float s2 = static_cast<float>(wrapped);
-  const float lhs = (m * s2 - s * s) - s * (m - 1.0f);
-  const float rhs = (s * 6.0f) * root;
-  return lhs > rhs;
-}
+// Stage 0's test (dext.cc:99-110) on exact window sums: count m (with m - 1,
+// both in the real type), sum S of the values and the sum of the squares s2
+// in the real type.  For uint16 frames s2 is the reference's wrapped uint32
+// sum Q; for uint32 frames Q is the sum of the squares' low 26 bits and H
+// that of their high parts, nonzero only near a pixel of 2^13 or more.  The
+// root is R(sqrt(2 (m - 1))), +inf below two, so that m < 2 tests false.
+template <typename T> struct Stage0;
+
+template <> struct Stage0<uint32_t> {
+  __device__ static __forceinline__ double squares(uint32_t Q) {
+    return static_cast<double>(Q);
+  }
+  __device__ static __forceinline__ double squares(uint32_t Q, uint32_t H) {
+    return __fma_rn(static_cast<double>(H), 0x1p26, static_cast<double>(Q));
+  }
+  __device__ static __forceinline__ bool test(double m, double mm1, uint32_t S,
+                                              double s2, double root) {
+    const double s = static_cast<double>(S);
+    // (m * s2 - s * s) - s * (m - 1): every term and partial result is an
+    // integer below 2^53, so this one-rounding form is the same exact value.
+    const double lhs = __fma_rn(m, s2, -(s * (s + mm1)));
+    const double rhs = (s * 6.0) * root;
+    return lhs > rhs;
+  }
+};
+
+template <> struct Stage0<uint16_t> {
+  // The reference's accumulator is uint32 and wraps; Q is that wrapped sum.
+  __device__ static __forceinline__ float squares(uint32_t Q) {
+    return static_cast<float>(Q);
+  }
+  __device__ static __forceinline__ float squares(uint32_t Q, uint32_t) {
+    return static_cast<float>(Q);
+  }
+  __device__ static __forceinline__ bool test(float m, float mm1, uint32_t S,
+                                              float s2, float root) {
+    const float s = static_cast<float>(S);
+    const float lhs = (m * s2 - s * s) - s * mm1;
+    const float rhs = (s * 6.0f) * root;
+    return lhs > rhs;
+  }
+};
 
 // What one wave keeps in LDS: the rings of raw rows (the low ring column by
 // column, so that a lane's 16-bit reads fall in distinct banks) and the
@@ -295,13 +340,16 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
                                       const typename traits<T>::R *table) {
   using R = typename traits<T>::R;
   using Wt = typename traits<T>::W;
+  using S0 = Stage0<T>;
   constexpr int P = KQ / 4;          // nibble histories, one per 4 columns
   constexpr int kOut = out_columns(KQ);
+  constexpr bool kU32 = sizeof(T) == 4;
+  constexpr uint32_t kAll = (1u << KQ) - 1u;
   const int height = static_cast<int>(a.height);
   const int width = static_cast<int>(a.width);
   const int strip = static_cast<int>(item % a.strips);
   const int seg = static_cast<int>(item / a.strips);
-  const int x0 = strip * kOut;
+  const int x0 = strip * kOut - kPhase;
   const int c0 = x0 - kHalo + KQ * static_cast<int>(lane); // first column
   const int y0 = seg * static_cast<int>(a.seg_rows);
   const int y1 = min(height, y0 + static_cast<int>(a.seg_rows));
@@ -317,10 +365,10 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       v[q] = ring[s][q][lane];
   };
   const uint32_t masked = traits<T>::masked;
-  // A row whose pixels are all below this holds no masked pixel and none
-  // that needs the high ring or the exact path.
-  const uint32_t plain = sizeof(T) == 4 ? kWide : masked;
-  const R root49 = table[49];
+  // A row whose pixels are all below this holds no masked pixel, none that
+  // needs the high ring or the exact path, and (uint32 frames) no square of
+  // 2^26 or more.
+  const uint32_t plain = kU32 ? kHot : masked;
 
   // Stage-2 candidates wait in LDS, kStage (one a lane) at a time, each as
   // {index, value | mask bit << 18, count | sum word, mask word}.  A flush
@@ -432,7 +480,12 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       ring[s][q][lane] = 0;
 
   uint32_t v0w[KQ] = {}; // stage 0 vertical: sum
-  double v0s[KQ] = {};   // stage 0 vertical: sum of squares
+  // Stage 0 vertical sums of squares: uint16 frames, the wrapped uint32 sum
+  // (the reference's own accumulator); uint32 frames, the squares' low 26
+  // bits (49 of them stay below 2^32) and, beside, their high parts, nonzero
+  // only while a row with a pixel of 2^13 or more is in the window.
+  uint32_t v0q[KQ] = {};
+  uint32_t v0h[KQ] = {};
   uint32_t v2w[KQ] = {}; // stage 2 vertical: sum
   // Per four columns (columns 4h .. 4h + 3 of the lane):
   //   mh[h]  masked nibbles, row y_in - k at nibble k (k < 8); rows before
@@ -450,63 +503,87 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     hr[h] = 0;
   }
   // Wave-uniform row records, row y_in - k at bit k once y_in is in: rows
-  // with a pixel at or above 2^16; rows (y_in - 5) - k with an unmasked scr1
-  // pixel anywhere in the wave.
-  uint32_t wide_rows = 0, x1_rows = 0;
+  // with a pixel at or above 2^16 (which the high ring holds); rows with a
+  // pixel at or above 2^13 (uint32 frames: their squares are split); rows
+  // (y_in - 5) - k with an unmasked scr1 pixel anywhere in the wave.
+  uint32_t wide_rows = 0, hot_rows = 0, x1_rows = 0;
   // Wave-uniform: rows (y_in - 3) - k with a horizontal-erode bit anywhere in
   // the wave, at bit k.
   uint32_t y_rows = 0;
   // Wave-uniform: those rows with such a bit at a pixel unmasked in its row.
   uint32_t yu_rows = 0;
 
-  // Masks that stay put (module gaps, the frame's edges, dead columns): while
-  // every lane's masked columns are the same in rows y_in-6 .. y_in, the
-  // lattice pre-test still applies with one column per four, `sq[h]` of
-  // columns 4h .. 4h + 3: column 2 if it is unmasked, else 1, else 3.  The
-  // argument of the unmasked case carries over when every unmasked column
-  // whose erode reaches an output has such a column within two of it (a
-  // column unmasked in the row tested is unmasked in every row of the
-  // erodes through it), which the plan checks; the counts are then fixed,
-  // sm[h] valid pixels in column sq[h]'s window, with its table root.
-  uint32_t s_pat = ~0u; // the pattern planned for (per lane)
-  bool s_ok = false;    // wave-uniform: the plan covers every column
+  // Masks that stay put.  While every lane's masked columns are the same in
+  // the 7 rows of stage 0's window (no masked pixel at all being the common
+  // case), each column's count is fixed: sm[q] = 7 x its window's unmasked
+  // columns.  The plan holds them, and for the lattice columns 4h + 2 of
+  // every lane their counts and roots in the real type.  The lattice
+  // pre-test applies when every unmasked column whose erode reaches an
+  // output has an unmasked lattice column within two of it (s_ok): a column
+  // unmasked in the row tested is unmasked in every row of the erodes
+  // through it, so each such erode window then holds an unmasked lattice
+  // pixel in each of its rows, and when all of those test negative no such
+  // window is all scr0, whatever the row's other pixels test.
+  uint32_t s_pat = 0;   // the pattern planned for (per lane)
   uint32_t s_y = 0;     // the horizontal erode of the masked columns alone
-  uint32_t sq[P], sm[P];
-  R sroot[P];
+  uint32_t sm[KQ];
+  R lat_m[P], lat_mm1[P], lat_root[P];
+  uint32_t s_ok = 0, s_yany = 0, s_all49 = 0; // wave-uniform
+#pragma unroll
+  for (int q = 0; q < KQ; q++)
+    sm[q] = 0;
 #pragma unroll
   for (int h = 0; h < P; h++) {
-    sq[h] = 0;
-    sm[h] = 0;
-    sroot[h] = table[0];
+    lat_m[h] = R(0);
+    lat_mm1[h] = R(-1);
+    lat_root[h] = table[0];
   }
-  auto plan_static = [&](uint32_t pat) {
+  auto plan = [&](uint32_t pat) {
     s_pat = pat;
-    // bit j: column c0 - 3 + j
+    // bit j: column c0 - 3 + j masked
     const uint32_t x = ((from_left(pat) >> (KQ - 3)) & 7u) | (pat << 3) |
                        ((from_right(pat) & 7u) << (KQ + 3));
-    uint32_t own = 0;
+    bool not49 = false;
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+      sm[q] = 7u * (7u - static_cast<uint32_t>(__popc((x >> q) & 0x7Fu)));
+      not49 |= sm[q] != 49u;
+    }
+    s_all49 = bit_any(not49) ^ 1u;
+    uint32_t lat = 0; // unmasked lattice columns of this lane
 #pragma unroll
     for (int h = 0; h < P; h++) {
-      const uint32_t g = (pat >> (4 * h)) & 0xFu;
-      const uint32_t q = !(g & 4u) ? 2u : !(g & 2u) ? 1u : !(g & 8u) ? 3u : 0u;
-      const uint32_t col = 4u * h + q;
-      const uint32_t m = q ? 7u * (7u - static_cast<uint32_t>(
-                                            __popc((x >> col) & 0x7Fu)))
-                           : 0u;
-      sq[h] = q;
-      sm[h] = m;
-      sroot[h] = table[m];
-      own |= (q ? 1u : 0u) << col;
+      const uint32_t m = sm[4 * h + 2];
+      lat_m[h] = static_cast<R>(m);
+      lat_mm1[h] = static_cast<R>(m) - R(1);
+      lat_root[h] = table[m];
+      lat |= ((pat >> (4 * h + 2)) & 1u ? 0u : 1u) << (4 * h + 2);
     }
-    // bit j: column c0 - 4 + j holds a tested column
-    const uint32_t t = (from_left(own) >> (KQ - 4)) | (own << 4) |
-                       (from_right(own) << (KQ + 4));
+    // bit j: column c0 - 4 + j is an unmasked lattice column
+    const uint32_t t = (from_left(lat) >> (KQ - 4)) | (lat << 4) |
+                       (from_right(lat) << (KQ + 4));
     const uint32_t near = t | (t >> 1) | (t >> 2) | (t >> 3) | (t >> 4);
-    const uint32_t need = ~pat & reach;
-    s_ok = __ballot(((near >> 2) & need) != need) == 0;
+    const uint32_t need = ~pat & reach & kAll;
+    s_ok = bit_any(((near >> 2) & need) != need) ^ 1u;
     const uint32_t e = ((from_left(pat) >> (KQ - 2)) & 3u) | (pat << 2) |
                        ((from_right(pat) & 3u) << (KQ + 2));
-    s_y = e & (e >> 1) & (e >> 2) & (e >> 3) & (e >> 4) & ((1u << KQ) - 1u);
+    s_y = e & (e >> 1) & (e >> 2) & (e >> 3) & (e >> 4) & kAll;
+    s_yany = bit_any(s_y != 0u);
+  };
+  // The plan's state, one wave-uniform word: bits 0-3 count the rows still
+  // to come before the masked columns have stayed put through a whole
+  // stage-0 window (zero once they have), bit 4 says the plan is older than
+  // them.  `irregular` says the previous row had masked pixels beyond the
+  // frame's edges (its masked columns are then the history's newest
+  // nibbles); the rows before the first count as all masked.
+  uint32_t pstate = 6u | 16u;
+  uint32_t irregular = 1;
+  auto last_mbits = [&]() {
+    uint32_t m = 0;
+#pragma unroll
+    for (int h = 0; h < P; h++)
+      m |= (mh[h] & 0xFu) << (4 * h);
+    return m;
   };
 
   const int steps = (y1 - y0) + 20;
@@ -514,10 +591,14 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
   // while the row kAhead below loads into the one read the step before, so
   // no value is copied out of the way of a load.  Rows outside the frame
   // load a clamped row (their values are replaced below), and rows past the
-  // segment's last reload that last row, from the cache.
+  // segment's last reload that last row, from the cache.  Columns start at a
+  // multiple of four, so a lane is wholly inside or wholly outside the
+  // frame unless the width is not one (`straddle`).
   const int cl = min(max(c0, 0), width - KQ);
   const int sh = c0 - cl;
-  const bool edge = __ballot(outside != 0) != 0; // wave-uniform
+  const bool edge = __ballot(outside != 0) != 0;           // wave-uniform
+  const bool lane_out = outside == kAll;
+  const bool straddle = __ballot(outside != 0 && !lane_out) != 0;
   const int y_last = min(height, y1 + 10) - 1;
   auto issue = [&](int y, uint32_t(&v)[KQ]) {
     load_lane<T, KQ>(frame, width, max(0, min(y, y_last)), cl, v);
@@ -526,21 +607,41 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
 #pragma unroll
   for (int k = 0; k < kAhead; k++)
     issue(y0 - 10 + k, ahead[k]);
+  // Stage 2's entering rows, whole (high bits added), kept for two more
+  // steps: the row entering stage 2 at step t (y_in - 5) is the row leaving
+  // stage 0 at step t + 2 (y_in + 2 - 7).  Rows before the first are zero.
+  uint32_t enter[3][KQ];
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int q = 0; q < KQ; q++)
+      enter[k][q] = 0;
   int slot = 0; // ring slot of the entering row
-  auto row = [&](int t, uint32_t(&val)[KQ], uint32_t(&next)[KQ]) {
+  auto row = [&](int t, uint32_t(&val)[KQ], uint32_t(&next)[KQ],
+                 uint32_t(&in2)[KQ], const uint32_t(&out0)[KQ]) {
     const int y_in = y0 - 10 + t;
 
     // ---- the entering row: values with masked as zero, masked bits ----
     uint32_t mbits = outside;
-    bool wide = false;
-    if (y_in < 0 || y_in >= height) {
+    uint32_t wide = 0, hot = 0; // wave-uniform
+    uint32_t changed;           // wave-uniform: masked columns moved
+    if (static_cast<uint32_t>(y_in) >= static_cast<uint32_t>(height)) {
 #pragma unroll
       for (int q = 0; q < KQ; q++)
         val[q] = 0;
-      mbits = (1u << KQ) - 1u;
+      mbits = kAll;
+      changed = bit_any(mbits != (irregular ? last_mbits() : outside));
+      irregular = bit_any(mbits != outside);
     } else {
-      if (edge)
-        shift_lane<KQ>(val, sh, outside);
+      if (edge) {
+        if (straddle) {
+          shift_lane<KQ>(val, sh, outside);
+        } else {
+#pragma unroll
+          for (int q = 0; q < KQ; q++)
+            val[q] = lane_out ? 0u : val[q];
+        }
+      }
       uint32_t top = 0;
 #pragma unroll
       for (int q = 0; q < KQ; q++)
@@ -549,70 +650,51 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
         // Masked pixels become zero; a valid pixel at or above kFast voids
         // this frame's fast sums (the caller re-runs it exactly) and is
         // clamped so that no sum spills into its count bits.
-        uint32_t any = 0;
+        uint32_t vmax = 0;
 #pragma unroll
         for (int q = 0; q < KQ; q++) {
           const bool m = val[q] >= masked;
           mbits |= (m ? 1u : 0u) << q;
           big |= !m && val[q] >= kFast;
           val[q] = m ? 0u : min(val[q], kFast - 1u);
-          any |= val[q];
+          vmax = max(vmax, val[q]);
         }
-        wide = __ballot(any >= kWide) != 0;
+        if constexpr (kU32) {
+          wide = bit_any(vmax >= kWide);
+          hot = bit_any(vmax >= kHot);
+        }
+        changed = bit_any(mbits != (irregular ? last_mbits() : outside));
+        irregular = bit_any(mbits != outside);
+      } else {
+        changed = irregular;
+        irregular = 0;
       }
     }
     issue(y_in + kAhead, next);
+    if (__builtin_expect((changed | pstate) != 0u, 0))
+      pstate = changed ? (6u | 16u) : pstate - min(pstate & 15u, 1u);
 
-    // Ring rows leaving stage 0 (y_in - 7), entering stage 2 (y_in - 5) and
-    // leaving it (y_in - 16, whose slot the entering row takes once it has
-    // been read), low 16 bits.
-    const int s7 = (slot - 7) & (kRing - 1);
+    // Ring rows entering stage 2 (y_in - 5) and leaving it (y_in - 16, whose
+    // slot the entering row takes once it has been read), low 16 bits; the
+    // row leaving stage 0 (y_in - 7) is stage 2's entering row of two steps
+    // back, whole.  Before row y_in is in, row y_in - k is bit k - 1 of the
+    // row records.  (The change to stage 2's sums is formed in the block of
+    // the reads, so that the 16-bit loads' zero extension folds into them.)
     const int s5 = (slot - 5) & (kRing - 1);
     const int s16 = slot;
-    // (Read in the same block as their first use, so that the 16-bit loads'
-    // zero extension folds into them.)
-    uint32_t out0[KQ], in2[KQ], out2[KQ];
-    ring_row(s7, out0);
+    uint32_t out2[KQ], d2[KQ];
     ring_row(s5, in2);
     ring_row(s16, out2);
+#pragma unroll
+    for (int q = 0; q < KQ; q++)
+      d2[q] = in2[q] - out2[q];
 #pragma unroll
     for (int q = 0; q < KQ; q++)
       ring[slot][q][lane] = static_cast<uint16_t>(val[q]);
 #pragma unroll
     for (int h = 0; h < P; h++)
       mh[h] = (mh[h] << 4) | ((mbits >> (4 * h)) & 0xFu);
-
-    // ---- stage 0: vertical 7-row sums (rows y_in-6 .. y_in) ----
-#pragma unroll
-    for (int q = 0; q < KQ; q++) {
-      const uint32_t d = val[q] - out0[q];
-      v0w[q] += d;
-      v0s[q] = __fma_rn(static_cast<double>(static_cast<int32_t>(d)),
-                        static_cast<double>(val[q] + out0[q]), v0s[q]);
-    }
-    // Stage 2's entering and leaving rows, as a change to its sums.
-    uint32_t d2[KQ];
-#pragma unroll
-    for (int q = 0; q < KQ; q++)
-      d2[q] = in2[q] - out2[q];
-
-    // The ring rows above hold only the low 16 bits.  For the rare rows with
-    // a pixel at or above 2^16 (before row y_in is in, row y_in - k is bit
-    // k - 1 of wide_rows), add each such pixel's high part H to what used
-    // it: stage 0's sums lose H more and their squares (o + H)^2 - o^2 more;
-    // stage 2's entering and leaving values gain H.
-    if (__builtin_expect((wide_rows & ((1u << 6) | (1u << 4) | (1u << 15))) != 0,
-                         0)) {
-      if ((wide_rows >> 6) & 1u) {
-        uint32_t h[KQ] = {};
-        add_high<KQ>(high[s7][lane], h);
-#pragma unroll
-        for (int q = 0; q < KQ; q++) {
-          v0w[q] -= h[q];
-          v0s[q] -= static_cast<double>(h[q]) *
-                    static_cast<double>(2u * out0[q] + h[q]);
-        }
-      }
+    if (__builtin_expect((wide_rows & ((1u << 4) | (1u << 15))) != 0, 0)) {
       if ((wide_rows >> 4) & 1u)
         add_high<KQ>(high[s5][lane], in2);
       if ((wide_rows >> 15) & 1u)
@@ -621,134 +703,139 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       for (int q = 0; q < KQ; q++)
         d2[q] = in2[q] - out2[q];
     }
-    wide_rows = (wide_rows << 1) | (wide ? 1u : 0u);
-    if (__builtin_expect(wide, 0))
+
+    // ---- stage 0: vertical 7-row sums (rows y_in-6 .. y_in) ----
+    // (The change of the squares is formed in either branch and added after
+    // them, so that no sum is stored from two places.)
+    uint32_t dq[KQ];
+    if (!kU32 || __builtin_expect((hot | ((hot_rows >> 6) & 1u)) == 0u, 1)) {
+      // Every pixel entering or leaving is below 2^13 (uint16 frames: below
+      // 2^16, and the product wraps as the reference's sum does), so the
+      // change of the squares is one 24-bit product.
+#pragma unroll
+      for (int q = 0; q < KQ; q++)
+        dq[q] = static_cast<uint32_t>(__mul24(static_cast<int>(val[q] - out0[q]),
+                                              static_cast<int>(val[q] + out0[q])));
+    } else {
+      // A pixel of 2^13 or more enters or leaves (uint32 frames only): the
+      // squares split at bit 26.
+#pragma unroll
+      for (int q = 0; q < KQ; q++) {
+        const uint64_t pi = static_cast<uint64_t>(val[q] & (kFast - 1u)) *
+                            (val[q] & (kFast - 1u));
+        const uint64_t po = static_cast<uint64_t>(out0[q] & (kFast - 1u)) *
+                            (out0[q] & (kFast - 1u));
+        dq[q] = (static_cast<uint32_t>(pi) & (kSplit - 1u)) -
+                (static_cast<uint32_t>(po) & (kSplit - 1u));
+        v0h[q] += static_cast<uint32_t>(pi >> 26) - static_cast<uint32_t>(po >> 26);
+      }
+    }
+#pragma unroll
+    for (int q = 0; q < KQ; q++) {
+      v0w[q] += val[q] - out0[q];
+      v0q[q] += dq[q];
+    }
+    wide_rows = (wide_rows << 1) | wide;
+    hot_rows = (hot_rows << 1) | hot;
+    if (__builtin_expect(wide != 0u, 0))
       high[slot][lane] = static_cast<typename WaveLds<KQ>::High>(pack_high<KQ>(val));
 
     // ---- stage 0: 7 columns across lanes, test -> scr0 of row y_in-3 ----
     uint32_t scr0 = 0; // starts as the masked bits of row y_in-3
-    uint32_t m7any = 0;
-    bool quiet = false; // wave-uniform: scr0 is a planned static mask alone
 #pragma unroll
-    for (int h = 0; h < P; h++) {
+    for (int h = 0; h < P; h++)
       scr0 |= ((mh[h] >> 12) & 0xFu) << (4 * h);
-      m7any |= mh[h] & 0x0FFFFFFFu;
-    }
     const uint32_t scr0_masked = scr0;
-    // A masked pixel in rows y_in-6 .. y_in of any lane: counts vary and
-    // travel in the top bits of the sums; otherwise every count is 49.
-    if (__ballot(m7any != 0) == 0) {
-      // Lattice first: the columns 4k + 2 of every lane.  Every 5 x 5 erode
-      // window that reaches an output through stage 2 lies within the strip
-      // and holds one such column in each of its rows, and here no pixel of
-      // the row is masked; so when every lattice pixel of the row tests
-      // negative, no such window is all scr0 whatever the row's other
-      // pixels test, and they are left clear.  Otherwise the other columns'
-      // sums follow from the lattice columns' by the columns entering and
-      // leaving.  Column i of the row (i from -3 to KQ + 2, the outer ones
-      // the neighbouring lanes') is at [i + 3].
-      double es[KQ + 6];
-      uint32_t ew[KQ + 6];
-#pragma unroll
-      for (int q = 0; q < KQ; q++) {
-        es[q + 3] = v0s[q];
-        ew[q + 3] = v0w[q];
-      }
-      es[2] = from_left(v0s[KQ - 1]);
-      ew[2] = from_left(v0w[KQ - 1]);
-      es[KQ + 3] = from_right(v0s[0]);
-      ew[KQ + 3] = from_right(v0w[0]);
-      es[KQ + 4] = from_right(v0s[1]);
-      ew[KQ + 4] = from_right(v0w[1]);
-      double hs[KQ];
-      uint32_t hw[KQ];
-      uint32_t lattice = 0;
-#pragma unroll
-      for (int q = 2; q < KQ; q += 4) {
-        hs[q] = tree_sum<7>(es + q);
-        hw[q] = tree_sum<7>(ew + q);
-        if (stage0_test<T>(49u, hw[q], hs[q], root49))
-          lattice |= 1u << q;
-      }
-      if (__ballot(lattice != 0) != 0) {
-        es[1] = from_left(v0s[KQ - 2]);
-        ew[1] = from_left(v0w[KQ - 2]);
-        es[0] = from_left(v0s[KQ - 3]);
-        ew[0] = from_left(v0w[KQ - 3]);
-        es[KQ + 5] = from_right(v0s[2]);
-        ew[KQ + 5] = from_right(v0w[2]);
-#pragma unroll
-        for (int q = 2; q < KQ; q += 4) {
-          hs[q - 1] = (hs[q] - es[q + 6]) + es[q - 1];
-          hw[q - 1] = (hw[q] - ew[q + 6]) + ew[q - 1];
-          hs[q - 2] = (hs[q - 1] - es[q + 5]) + es[q - 2];
-          hw[q - 2] = (hw[q - 1] - ew[q + 5]) + ew[q - 2];
-          hs[q + 1] = (hs[q] - es[q]) + es[q + 7];
-          hw[q + 1] = (hw[q] - ew[q]) + ew[q + 7];
+    // Rows above y0 - 7 are never read by an output's windows: their stage 0
+    // is skipped (scr0 the masked pixels, no erode bits), which changes
+    // nothing that reaches an output.  A pixel of 2^13 or more in rows
+    // y_in-6 .. y_in (uint32 frames) brings the squares' high parts in.
+    bool quiet = true;
+    if (t >= 6) {
+      const bool hwin = kU32 && (hot_rows & 0x7Fu) != 0u;
+      if ((pstate & 15u) == 0u) {
+        // Masks that stay put (or none): the plan's counts.
+        if (__builtin_expect(pstate != 0u, 0)) {
+          plan(mbits);
+          pstate = 0;
         }
-        scr0 |= lattice;
-#pragma unroll
-        for (int q = 0; q < KQ; q++)
-          if ((q & 3) != 2 && stage0_test<T>(49u, hw[q], hs[q], root49))
-            scr0 |= 1u << q;
-      }
-    } else {
-      // Masks that stay put take the pre-test on each lane's planned columns.
-      bool counted = true;
-      uint32_t pat = 0;
-      bool moving = false;
-#pragma unroll
-      for (int h = 0; h < P; h++) {
-        const uint32_t nib = mh[h] & 0xFu;
-        moving |= (mh[h] & 0x0FFFFFFFu) != nib * 0x1111111u;
-        pat |= nib << (4 * h);
-      }
-      if (__ballot(moving) == 0) {
-        if (__ballot(pat != s_pat) != 0)
-          plan_static(pat);
-        if (s_ok) {
-          // Column i of the row (from -3 to KQ + 2) at [i + 3].
-          double es[KQ + 6];
-          uint32_t ew[KQ + 6];
-          gather<KQ, 3>(v0s, es, std::make_integer_sequence<int, KQ + 6>{});
-          gather<KQ, 3>(v0w, ew, std::make_integer_sequence<int, KQ + 6>{});
-          bool hit = false;
+        uint32_t hw[KQ], hq[KQ], hh[KQ];
+        lattice_sums<3>(v0w, hw, std::make_integer_sequence<int, P>{});
+        lattice_sums<3>(v0q, hq, std::make_integer_sequence<int, P>{});
+        uint32_t lattice = 0;
+        if (__builtin_expect(!hwin, 1)) {
 #pragma unroll
           for (int h = 0; h < P; h++) {
-            const int b = 4 * h;
-            // the windows of columns b + 2, b + 1 and b + 3
-            const double s2 = tree_sum<7>(es + b + 2);
-            const uint32_t w2 = tree_sum<7>(ew + b + 2);
-            const double s1 = (s2 - es[b + 8]) + es[b + 1];
-            const uint32_t w1 = (w2 - ew[b + 8]) + ew[b + 1];
-            const double s3 = (s2 - es[b + 2]) + es[b + 9];
-            const uint32_t w3 = (w2 - ew[b + 2]) + ew[b + 9];
-            const double ss = sq[h] == 1u ? s1 : (sq[h] == 3u ? s3 : s2);
-            const uint32_t ww = sq[h] == 1u ? w1 : (sq[h] == 3u ? w3 : w2);
-            hit |= stage0_test<T>(sm[h], ww, ss, sroot[h]);
+            const int c = 4 * h + 2;
+            if (S0::test(lat_m[h], lat_mm1[h], hw[c], S0::squares(hq[c]),
+                         lat_root[h]))
+              lattice |= 1u << c;
           }
-          // No planned column positive: scr0 is the masked pixels alone.
-          counted = __ballot(hit) != 0;
-          quiet = !counted;
+        } else {
+          lattice_sums<3>(v0h, hh, std::make_integer_sequence<int, P>{});
+#pragma unroll
+          for (int h = 0; h < P; h++) {
+            const int c = 4 * h + 2;
+            if (S0::test(lat_m[h], lat_mm1[h], hw[c],
+                         S0::squares(hq[c], hh[c]), lat_root[h]))
+              lattice |= 1u << c;
+          }
         }
-      }
-      if (counted) {
-        double hs[KQ];
-        across<3>(v0s, hs);
-        uint32_t word[KQ], hw[KQ];
-        // each column's unmasked pixels in rows y_in-6 .. y_in, counted
-        // directly from the clear bits of the masked history
+        if (!s_ok || __ballot((lattice & ~s_pat) != 0) != 0) {
+          quiet = false;
+          scr0 |= lattice;
+          slide_all<3>(v0w, hw, std::make_integer_sequence<int, P>{});
+          slide_all<3>(v0q, hq, std::make_integer_sequence<int, P>{});
+          if (__builtin_expect(!hwin, 1)) {
+            if (s_all49) {
+              const R root49 = table[49];
+#pragma unroll
+              for (int q = 0; q < KQ; q++)
+                if ((q & 3) != 2 &&
+                    S0::test(R(49), R(48), hw[q], S0::squares(hq[q]), root49))
+                  scr0 |= 1u << q;
+            } else {
+#pragma unroll
+              for (int q = 0; q < KQ; q++) {
+                const R m = static_cast<R>(sm[q]);
+                if ((q & 3) != 2 &&
+                    S0::test(m, m - R(1), hw[q], S0::squares(hq[q]), table[sm[q]]))
+                  scr0 |= 1u << q;
+              }
+            }
+          } else {
+            slide_all<3>(v0h, hh, std::make_integer_sequence<int, P>{});
+#pragma unroll
+            for (int q = 0; q < KQ; q++) {
+              const R m = static_cast<R>(sm[q]);
+              if ((q & 3) != 2 && S0::test(m, m - R(1), hw[q],
+                                           S0::squares(hq[q], hh[q]), table[sm[q]]))
+                scr0 |= 1u << q;
+            }
+          }
+        }
+      } else {
+        // Masks that move: each column's unmasked pixels in rows
+        // y_in-6 .. y_in counted from the clear bits of the masked history,
+        // carried in the top bits of the sums.
+        quiet = false;
+        uint32_t word[KQ], hw[KQ], hq[KQ], hh[KQ];
 #pragma unroll
         for (int q = 0; q < KQ; q++)
           word[q] = v0w[q] +
                     (static_cast<uint32_t>(__popc(
                          ~mh[q / 4] & static_cast<uint32_t>(column_bits(0, 7, q % 4))))
                      << 26);
-        across<3>(word, hw);
+        row_sums<3>(word, hw);
+        row_sums<3>(v0q, hq);
+        if (hwin)
+          row_sums<3>(v0h, hh);
 #pragma unroll
         for (int q = 0; q < KQ; q++) {
           const uint32_t m = hw[q] >> 26;
-          if (stage0_test<T>(m, hw[q] & (kCount0 - 1), hs[q], table[m]))
+          const R mr = static_cast<R>(m);
+          const auto s2 = hwin ? S0::squares(hq[q], hh[q]) : S0::squares(hq[q]);
+          if (S0::test(mr, mr - R(1), hw[q] & (kCount0 - 1), s2, table[m]))
             scr0 |= 1u << q;
         }
       }
@@ -758,23 +845,28 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     uint32_t x1 = 0, ex = 0; // scr1 and not masked; scr1 or masked
     {
       // The horizontal erode of row y_in-3 (bit i: columns c0 - 2 + i ..
-      // c0 + 2 + i all in scr0), left clear when no lane has a scr0 pixel
-      // in the row and taken from the plan when scr0 is its static mask.
+      // c0 + 2 + i all in scr0): none for skipped rows, the plan's for rows
+      // whose scr0 is its static mask, none where no lane has a scr0 pixel.
       const uint32_t m3 = scr0_masked;
-      uint32_t y = 0;
+      uint32_t y = 0, ybit = 0, yubit = 0;
       if (quiet) {
-        y = s_y;
+        if (t >= 6) {
+          y = s_y;
+          ybit = s_yany;
+        }
       } else if (__ballot(scr0 != 0) != 0) {
         const uint32_t left = from_left(scr0);
         const uint32_t right = from_right(scr0);
         // bit i is column c0 - 2 + i
         const uint32_t x = ((left >> (KQ - 2)) & 3u) | (scr0 << 2) |
                            ((right & 3u) << (KQ + 2));
-        y = x & (x >> 1) & (x >> 2) & (x >> 3) & (x >> 4) & ((1u << KQ) - 1u);
+        y = x & (x >> 1) & (x >> 2) & (x >> 3) & (x >> 4) & kAll;
+        ybit = bit_any(y != 0);
+        // and of them those at a pixel unmasked in the row (a positive one)
+        yubit = bit_any((y & ~m3) != 0);
       }
-      y_rows = (y_rows << 1) | (__ballot(y != 0) != 0 ? 1u : 0u);
-      // and of them those at a pixel unmasked in the row (a positive one)
-      yu_rows = (yu_rows << 1) | (!quiet && __ballot((y & ~m3) != 0) != 0 ? 1u : 0u);
+      y_rows = (y_rows << 1) | ybit;
+      yu_rows = (yu_rows << 1) | yubit;
 #pragma unroll
       for (int h = 0; h < P; h++)
         hr[h] = (hr[h] << 4) | ((y >> (4 * h)) & 0xFu);
@@ -793,7 +885,7 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
           x1 |= (all5 & ~m5) << (4 * h);
           ex |= (all5 | m5) << (4 * h);
         }
-        x1_rows = (x1_rows << 1) | (__ballot(x1 != 0) != 0 ? 1u : 0u);
+        x1_rows = (x1_rows << 1) | bit_any(x1 != 0);
       } else {
 #pragma unroll
         for (int h = 0; h < P; h++)
@@ -840,63 +932,65 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
 #pragma unroll
       for (int h = 0; h < P; h++)
         maybe |= (static_cast<uint32_t>(xh[h] >> 20) & 0xFu) << (4 * h);
-      maybe &= in_out;
-      // Counts: the pixels of rows y_in-15 .. y_in-5 not excluded, from the
-      // clear bits of the exclusion history.
-      uint32_t word[KQ];
-#pragma unroll
-      for (int q = 0; q < KQ; q++)
-        word[q] = v2w[q] + (static_cast<uint32_t>(__popcll(
-                                ~xh[q / 4] & column_bits(0, 11, q % 4)))
-                            << 25);
-      uint32_t hsum[KQ];
-      across<5>(word, hsum);
       const int s10 = (slot - 10) & (kRing - 1);
       uint32_t raw[KQ];
       ring_row(s10, raw);
       if ((wide_rows >> 10) & 1u)
         add_high<KQ>(high[s10][lane], raw);
-      uint32_t cand = 0;
+      uint32_t nz = 0;
 #pragma unroll
       for (int q = 0; q < KQ; q++)
-        cand |= (((maybe >> q) & 1u) && raw[q] > 0 ? 1u : 0u) << q;
-      auto entry = [&](int q) {
-        const int c = c0 + q;
-        return make_uint4(
-            static_cast<uint32_t>(y2) * a.width + static_cast<uint32_t>(c),
-            raw[q] | (static_cast<uint32_t>(c & 31) << 18), hsum[q],
-            static_cast<uint32_t>(y2) * a.wpr + static_cast<uint32_t>(c) / 32);
-      };
-      // One wave scan of the lanes' candidate counts places them all when
-      // the stage has room; a row with more goes column by column through
-      // put's chunks.
-      const uint32_t k = static_cast<uint32_t>(__popc(cand));
-      const uint32_t incl = wave_inclusive(k);
-      const uint32_t total = last_lane(incl);
-      if (staged + total <= kStage) {
-        uint32_t at = staged + incl - k;
+        nz |= min(raw[q], 1u) << q;
+      const uint32_t cand = maybe & in_out & nz;
+      if (__ballot(cand != 0) != 0) {
+        // Counts: the pixels of rows y_in-15 .. y_in-5 not excluded, from
+        // the clear bits of the exclusion history.
+        uint32_t word[KQ];
 #pragma unroll
         for (int q = 0; q < KQ; q++)
-          if ((cand >> q) & 1u)
-            stage[at++] = entry(q);
-        __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-        __builtin_amdgcn_wave_barrier();
-        staged += total;
-      } else {
+          word[q] = v2w[q] + (static_cast<uint32_t>(__popcll(
+                                  ~xh[q / 4] & column_bits(0, 11, q % 4)))
+                              << 25);
+        uint32_t hsum[KQ];
+        row_sums<5>(word, hsum);
+        auto entry = [&](int q) {
+          const int c = c0 + q;
+          return make_uint4(
+              static_cast<uint32_t>(y2) * a.width + static_cast<uint32_t>(c),
+              raw[q] | (static_cast<uint32_t>(c & 31) << 18), hsum[q],
+              static_cast<uint32_t>(y2) * a.wpr + static_cast<uint32_t>(c) / 32);
+        };
+        // One wave scan of the lanes' candidate counts places them all when
+        // the stage has room; a row with more goes column by column through
+        // put's chunks.
+        const uint32_t k = static_cast<uint32_t>(__popc(cand));
+        const uint32_t incl = wave_inclusive(k);
+        const uint32_t total = last_lane(incl);
+        if (staged + total <= kStage) {
+          uint32_t at = staged + incl - k;
 #pragma unroll
-        for (int q = 0; q < KQ; q++)
-          put(((cand >> q) & 1u) != 0, entry(q));
+          for (int q = 0; q < KQ; q++)
+            if ((cand >> q) & 1u)
+              stage[at++] = entry(q);
+          __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+          __builtin_amdgcn_wave_barrier();
+          staged += total;
+        } else {
+#pragma unroll
+          for (int q = 0; q < KQ; q++)
+            put(((cand >> q) & 1u) != 0, entry(q));
+        }
       }
     }
     slot = (slot + 1) & (kRing - 1);
   };
   static_assert(kAhead == 2, "the buffers rotate over three steps");
   for (int t = 0; t < steps; t += 3) {
-    row(t, ahead[0], ahead[2]);
+    row(t, ahead[0], ahead[2], enter[0], enter[1]);
     if (t + 1 < steps)
-      row(t + 1, ahead[1], ahead[0]);
+      row(t + 1, ahead[1], ahead[0], enter[1], enter[2]);
     if (t + 2 < steps)
-      row(t + 2, ahead[2], ahead[1]);
+      row(t + 2, ahead[2], ahead[1], enter[2], enter[0]);
   }
   flush();
   if (__ballot(big) != 0 && lane == 0)
@@ -910,7 +1004,8 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
 // item of each other.  The grid is the device's resident waves, so the launch
 // is persistent, and every wave leaves when the queue runs past the items.
 template <int Waves, int KQ>
-__global__ __launch_bounds__(Waves *kLanes) void fused_kernel(
+__global__ __launch_bounds__(Waves *kLanes)
+__attribute__((amdgpu_waves_per_eu(KQ == 4 ? 4 : 2))) void fused_kernel(
     const FusedParams *__restrict__ frames, uint32_t nframes,
     uint32_t total_items, const float *__restrict__ tf,
     const double *__restrict__ td, uint32_t *__restrict__ queue) {
@@ -996,7 +1091,7 @@ namespace sfi {
 
 uint32_t strips_of(uint32_t width, int lane_columns) {
   const uint32_t out = static_cast<uint32_t>(out_columns(lane_columns));
-  return (width + out - 1) / out;
+  return (width + kPhase + out - 1) / out;
 }
 
 uint32_t segment_rows(uint64_t strip_rows, const Tuning &t) {
