@@ -233,6 +233,10 @@ __device__ __forceinline__ void load_lane(const SF_GLOBAL T *__restrict__ frame,
 template <int KQ>
 __device__ __forceinline__ void shift_lane(uint32_t (&x)[KQ], int sh,
                                            uint32_t outside) {
+  // The shift's conditions are formed again for every row (the empty asm
+  // hides that they are the same each time): kept across the loop, they
+  // would hold a pair of scalar registers each.
+  __asm__("" : "+v"(sh), "+v"(outside));
   uint32_t r[KQ];
 #pragma unroll
   for (int q = 0; q < KQ; q++) {
@@ -323,6 +327,29 @@ template <int KQ> struct WaveLds {
   High high[kRing][kLanes];
   uint4 stage[kStage];
 };
+
+// The wave-uniform row records of a sweep, in one 64-bit word: each field is
+// a history of one bit a row, the newest at the field's first bit, and the
+// whole word shifts by one at the start of a row (kRecKeep clears the bit
+// each field's neighbour below shifts into it), so that every test is one
+// scalar AND.  Field by field, row r - k at bit k of the field:
+//   kRecW  rows with a pixel at or above 2^16, which the high ring holds;
+//          r = y_in, k <= 16;
+//   kRecH  rows with a pixel at or above 2^13 (uint32 frames: their squares
+//          are split); r = y_in, k <= 7;
+//   kRecY  rows with a horizontal-erode bit anywhere in the wave;
+//          r = y_in - 3, k <= 4;
+//   kRecU  of them, rows with such a bit at a pixel unmasked in its row;
+//          r = y_in - 3, k <= 2;
+//   kRecX  rows with an unmasked scr1 pixel anywhere in the wave;
+//          r = y_in - 5, k <= 11.
+constexpr int kRecW = 0, kRecH = 17, kRecY = 25, kRecU = 30, kRecX = 33;
+constexpr uint64_t kRecKeep = ~((1ull << kRecW) | (1ull << kRecH) |
+                                (1ull << kRecY) | (1ull << kRecU) |
+                                (1ull << kRecX));
+__host__ __device__ constexpr uint64_t rec_bit(int field, int k) {
+  return 1ull << (field + k);
+}
 
 // Bits of column q in rows of a nibble history: nibbles k0 .. k0 + n - 1.
 __host__ __device__ constexpr uint64_t column_bits(int k0, int n, int q) {
@@ -502,16 +529,9 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     xh[h] = ~0ull;
     hr[h] = 0;
   }
-  // Wave-uniform row records, row y_in - k at bit k once y_in is in: rows
-  // with a pixel at or above 2^16 (which the high ring holds); rows with a
-  // pixel at or above 2^13 (uint32 frames: their squares are split); rows
-  // (y_in - 5) - k with an unmasked scr1 pixel anywhere in the wave.
-  uint32_t wide_rows = 0, hot_rows = 0, x1_rows = 0;
-  // Wave-uniform: rows (y_in - 3) - k with a horizontal-erode bit anywhere in
-  // the wave, at bit k.
-  uint32_t y_rows = 0;
-  // Wave-uniform: those rows with such a bit at a pixel unmasked in its row.
-  uint32_t yu_rows = 0;
+  // The wave-uniform row records, one word (kRecW .. kRecX above), all
+  // shifted once at the start of a row.
+  uint64_t recs = 0;
 
   // Masks that stay put.  While every lane's masked columns are the same in
   // the 7 rows of stage 0's window (no masked pixel at all being the common
@@ -607,19 +627,11 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
 #pragma unroll
   for (int k = 0; k < kAhead; k++)
     issue(y0 - 10 + k, ahead[k]);
-  // Stage 2's entering rows, whole (high bits added), kept for two more
-  // steps: the row entering stage 2 at step t (y_in - 5) is the row leaving
-  // stage 0 at step t + 2 (y_in + 2 - 7).  Rows before the first are zero.
-  uint32_t enter[3][KQ];
-#pragma unroll
-  for (int k = 0; k < 3; k++)
-#pragma unroll
-    for (int q = 0; q < KQ; q++)
-      enter[k][q] = 0;
   int slot = 0; // ring slot of the entering row
-  auto row = [&](int t, uint32_t(&val)[KQ], uint32_t(&next)[KQ],
-                 uint32_t(&in2)[KQ], const uint32_t(&out0)[KQ]) {
+  auto row = [&](int t, uint32_t(&val)[KQ], uint32_t(&next)[KQ]) {
+    uint32_t in2[KQ], out0[KQ];
     const int y_in = y0 - 10 + t;
+    recs = (recs << 1) & kRecKeep;
 
     // ---- the entering row: values with masked as zero, masked bits ----
     uint32_t mbits = outside;
@@ -662,6 +674,8 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
         if constexpr (kU32) {
           wide = bit_any(vmax >= kWide);
           hot = bit_any(vmax >= kHot);
+          recs |= (static_cast<uint64_t>(wide) << kRecW) |
+                  (static_cast<uint64_t>(hot) << kRecH);
         }
         changed = bit_any(mbits != (irregular ? last_mbits() : outside));
         irregular = bit_any(mbits != outside);
@@ -683,6 +697,8 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     const int s5 = (slot - 5) & (kRing - 1);
     const int s16 = slot;
     uint32_t out2[KQ], d2[KQ];
+    const int s7 = (slot - 7) & (kRing - 1);
+    ring_row(s7, out0);
     ring_row(s5, in2);
     ring_row(s16, out2);
 #pragma unroll
@@ -694,10 +710,10 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
 #pragma unroll
     for (int h = 0; h < P; h++)
       mh[h] = (mh[h] << 4) | ((mbits >> (4 * h)) & 0xFu);
-    if (__builtin_expect((wide_rows & ((1u << 4) | (1u << 15))) != 0, 0)) {
-      if ((wide_rows >> 4) & 1u)
+    if (__builtin_expect((recs & (rec_bit(kRecW, 5) | rec_bit(kRecW, 16))) != 0, 0)) {
+      if (recs & rec_bit(kRecW, 5))
         add_high<KQ>(high[s5][lane], in2);
-      if ((wide_rows >> 15) & 1u)
+      if (recs & rec_bit(kRecW, 16))
         add_high<KQ>(high[s16][lane], out2);
 #pragma unroll
       for (int q = 0; q < KQ; q++)
@@ -708,7 +724,8 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     // (The change of the squares is formed in either branch and added after
     // them, so that no sum is stored from two places.)
     uint32_t dq[KQ];
-    if (!kU32 || __builtin_expect((hot | ((hot_rows >> 6) & 1u)) == 0u, 1)) {
+    if (!kU32 ||
+        __builtin_expect((recs & (rec_bit(kRecH, 0) | rec_bit(kRecH, 7))) == 0, 1)) {
       // Every pixel entering or leaving is below 2^13 (uint16 frames: below
       // 2^16, and the product wraps as the reference's sum does), so the
       // change of the squares is one 24-bit product.
@@ -719,6 +736,8 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     } else {
       // A pixel of 2^13 or more enters or leaves (uint32 frames only): the
       // squares split at bit 26.
+      if (recs & rec_bit(kRecW, 7))
+        add_high<KQ>(high[s7][lane], out0);
 #pragma unroll
       for (int q = 0; q < KQ; q++) {
         const uint64_t pi = static_cast<uint64_t>(val[q] & (kFast - 1u)) *
@@ -735,8 +754,6 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       v0w[q] += val[q] - out0[q];
       v0q[q] += dq[q];
     }
-    wide_rows = (wide_rows << 1) | wide;
-    hot_rows = (hot_rows << 1) | hot;
     if (__builtin_expect(wide != 0u, 0))
       high[slot][lane] = static_cast<typename WaveLds<KQ>::High>(pack_high<KQ>(val));
 
@@ -752,7 +769,7 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     // y_in-6 .. y_in (uint32 frames) brings the squares' high parts in.
     bool quiet = true;
     if (t >= 6) {
-      const bool hwin = kU32 && (hot_rows & 0x7Fu) != 0u;
+      const bool hwin = kU32 && (recs & (0x7Full << kRecH)) != 0;
       if ((pstate & 15u) == 0u) {
         // Masks that stay put (or none): the plan's counts.
         if (__builtin_expect(pstate != 0u, 0)) {
@@ -844,9 +861,6 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     // ---- stage 1: erode 5 x 5 -> scr1 of row y_in-5 ----
     uint32_t x1 = 0, ex = 0; // scr1 and not masked; scr1 or masked
     {
-      // The horizontal erode of row y_in-3 (bit i: columns c0 - 2 + i ..
-      // c0 + 2 + i all in scr0): none for skipped rows, the plan's for rows
-      // whose scr0 is its static mask, none where no lane has a scr0 pixel.
       const uint32_t m3 = scr0_masked;
       uint32_t y = 0, ybit = 0, yubit = 0;
       if (quiet) {
@@ -857,26 +871,19 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       } else if (__ballot(scr0 != 0) != 0) {
         const uint32_t left = from_left(scr0);
         const uint32_t right = from_right(scr0);
-        // bit i is column c0 - 2 + i
         const uint32_t x = ((left >> (KQ - 2)) & 3u) | (scr0 << 2) |
                            ((right & 3u) << (KQ + 2));
         y = x & (x >> 1) & (x >> 2) & (x >> 3) & (x >> 4) & kAll;
         ybit = bit_any(y != 0);
-        // and of them those at a pixel unmasked in the row (a positive one)
         yubit = bit_any((y & ~m3) != 0);
       }
-      y_rows = (y_rows << 1) | ybit;
-      yu_rows = (yu_rows << 1) | yubit;
+      recs |= (static_cast<uint64_t>(ybit) << kRecY) |
+              (static_cast<uint64_t>(yubit) << kRecU);
 #pragma unroll
       for (int h = 0; h < P; h++)
         hr[h] = (hr[h] << 4) | ((y >> (4 * h)) & 0xFu);
-      // scr1 = all5 | masked; a masked pixel adds zero to every sum, so
-      // only the unmasked part of scr1 needs excluding from the values.
-      // An unmasked scr1 pixel of row y_in-5 needs the horizontal erode in
-      // all five rows and, in its own row, at itself (so at a positive
-      // pixel); unless the rows have them somewhere in the wave, scr1 is
-      // just the masked pixels.
-      if ((y_rows & 0x1Fu) == 0x1Fu && ((yu_rows >> 2) & 1u)) {
+      if ((recs & (0x1Full << kRecY)) == (0x1Full << kRecY) &&
+          (recs & rec_bit(kRecU, 2))) {
 #pragma unroll
         for (int h = 0; h < P; h++) {
           const uint32_t all5 = hr[h] & (hr[h] >> 4) & (hr[h] >> 8) &
@@ -885,12 +892,11 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
           x1 |= (all5 & ~m5) << (4 * h);
           ex |= (all5 | m5) << (4 * h);
         }
-        x1_rows = (x1_rows << 1) | bit_any(x1 != 0);
+        recs |= static_cast<uint64_t>(bit_any(x1 != 0)) << kRecX;
       } else {
 #pragma unroll
         for (int h = 0; h < P; h++)
           ex |= ((mh[h] >> 20) & 0xFu) << (4 * h);
-        x1_rows <<= 1;
       }
     }
 
@@ -903,19 +909,21 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
 #pragma unroll
       for (int q = 0; q < KQ; q++)
         v2w[q] += d2[q];
-      if (x1_rows & 1u) {
+      if (__builtin_expect((recs & (rec_bit(kRecX, 0) | rec_bit(kRecX, 11))) != 0, 0)) {
+        if (recs & rec_bit(kRecX, 0)) {
 #pragma unroll
-        for (int q = 0; q < KQ; q++)
-          v2w[q] -= ((x1 >> q) & 1u) ? in2[q] : 0u;
-      }
-      if ((x1_rows >> 11) & 1u) {
-        uint32_t xout = 0; // row y_in-16's exclusion bits
+          for (int q = 0; q < KQ; q++)
+            v2w[q] -= ((x1 >> q) & 1u) ? in2[q] : 0u;
+        }
+        if (recs & rec_bit(kRecX, 11)) {
+          uint32_t xout = 0; // row y_in-16's exclusion bits
 #pragma unroll
-        for (int h = 0; h < P; h++)
-          xout |= (static_cast<uint32_t>(xh[h] >> 40) & 0xFu) << (4 * h);
+          for (int h = 0; h < P; h++)
+            xout |= (static_cast<uint32_t>(xh[h] >> 40) & 0xFu) << (4 * h);
 #pragma unroll
-        for (int q = 0; q < KQ; q++)
-          v2w[q] += ((xout >> q) & 1u) ? out2[q] : 0u;
+          for (int q = 0; q < KQ; q++)
+            v2w[q] += ((xout >> q) & 1u) ? out2[q] : 0u;
+        }
       }
 #pragma unroll
       for (int h = 0; h < P; h++)
@@ -923,7 +931,7 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     }
 
     // ---- stage 2: candidates of row y_in-10 ----
-    if (t >= 20 && ((x1_rows >> 5) & 1u)) {
+    if (__builtin_expect((recs & rec_bit(kRecX, 5)) != 0, 0) && t >= 20) {
       const int y2 = y_in - 10;
       // Excluded pixels of row y_in-10 in the output columns; of them the
       // candidates are the unmasked ones with a value above zero, and a
@@ -935,7 +943,7 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       const int s10 = (slot - 10) & (kRing - 1);
       uint32_t raw[KQ];
       ring_row(s10, raw);
-      if ((wide_rows >> 10) & 1u)
+      if (recs & rec_bit(kRecW, 10))
         add_high<KQ>(high[s10][lane], raw);
       uint32_t nz = 0;
 #pragma unroll
@@ -986,11 +994,11 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
   };
   static_assert(kAhead == 2, "the buffers rotate over three steps");
   for (int t = 0; t < steps; t += 3) {
-    row(t, ahead[0], ahead[2], enter[0], enter[1]);
+    row(t, ahead[0], ahead[2]);
     if (t + 1 < steps)
-      row(t + 1, ahead[1], ahead[0], enter[1], enter[2]);
+      row(t + 1, ahead[1], ahead[0]);
     if (t + 2 < steps)
-      row(t + 2, ahead[2], ahead[1], enter[2], enter[0]);
+      row(t + 2, ahead[2], ahead[1]);
   }
   flush();
   if (__ballot(big) != 0 && lane == 0)
