@@ -569,7 +569,7 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       sm[q] = 7u * (7u - static_cast<uint32_t>(__popc((x >> q) & 0x7Fu)));
       not49 |= sm[q] != 49u;
     }
-    s_all49 = bit_any(not49) ^ 1u;
+    s_all49 = __builtin_amdgcn_readfirstlane(bit_any(not49) ^ 1u);
     uint32_t lat = 0; // unmasked lattice columns of this lane
 #pragma unroll
     for (int h = 0; h < P; h++) {
@@ -584,11 +584,11 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
                        (from_right(lat) << (KQ + 4));
     const uint32_t near = t | (t >> 1) | (t >> 2) | (t >> 3) | (t >> 4);
     const uint32_t need = ~pat & reach & kAll;
-    s_ok = bit_any(((near >> 2) & need) != need) ^ 1u;
+    s_ok = __builtin_amdgcn_readfirstlane(bit_any(((near >> 2) & need) != need) ^ 1u);
     const uint32_t e = ((from_left(pat) >> (KQ - 2)) & 3u) | (pat << 2) |
                        ((from_right(pat) & 3u) << (KQ + 2));
     s_y = e & (e >> 1) & (e >> 2) & (e >> 3) & (e >> 4) & kAll;
-    s_yany = bit_any(s_y != 0u);
+    s_yany = __builtin_amdgcn_readfirstlane(bit_any(s_y != 0u));
   };
   // The plan's state, one wave-uniform word: bits 0-3 count the rows still
   // to come before the masked columns have stayed put through a whole
@@ -629,7 +629,6 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     issue(y0 - 10 + k, ahead[k]);
   int slot = 0; // ring slot of the entering row
   auto row = [&](int t, uint32_t(&val)[KQ], uint32_t(&next)[KQ]) {
-    uint32_t in2[KQ], out0[KQ];
     const int y_in = y0 - 10 + t;
     recs = (recs << 1) & kRecKeep;
 
@@ -688,75 +687,90 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     if (__builtin_expect((changed | pstate) != 0u, 0))
       pstate = changed ? (6u | 16u) : pstate - min(pstate & 15u, 1u);
 
-    // Ring rows entering stage 2 (y_in - 5) and leaving it (y_in - 16, whose
-    // slot the entering row takes once it has been read), low 16 bits; the
-    // row leaving stage 0 (y_in - 7) is stage 2's entering row of two steps
-    // back, whole.  Before row y_in is in, row y_in - k is bit k - 1 of the
-    // row records.  (The change to stage 2's sums is formed in the block of
-    // the reads, so that the 16-bit loads' zero extension folds into them.)
+    // Ring rows leaving stage 0 (y_in - 7), entering stage 2 (y_in - 5) and
+    // leaving it (y_in - 16, whose slot the entering row takes once it has
+    // been read), low 16 bits.  The changes to the sums are formed in the
+    // block of the reads (the empty asm holds them there), so that the
+    // 16-bit loads' zero extension folds into them, and the rare rows with a
+    // pixel at or above 2^13 or 2^16 correct the sums afterwards rather than
+    // redefine the changes, so that the common path copies nothing.
+    const int s7 = (slot - 7) & (kRing - 1);
     const int s5 = (slot - 5) & (kRing - 1);
     const int s16 = slot;
-    uint32_t out2[KQ], d2[KQ];
-    const int s7 = (slot - 7) & (kRing - 1);
+    uint32_t out0[KQ], in2[KQ], out2[KQ];
     ring_row(s7, out0);
     ring_row(s5, in2);
     ring_row(s16, out2);
 #pragma unroll
-    for (int q = 0; q < KQ; q++)
-      d2[q] = in2[q] - out2[q];
+    for (int q = 0; q < KQ; q++) {
+      uint32_t d0 = val[q] - out0[q];
+      uint32_t p0 = val[q] + out0[q];
+      uint32_t d2 = in2[q] - out2[q];
+      __asm__ volatile("" : "+v"(d0), "+v"(p0), "+v"(d2));
+      // ---- stage 0: vertical 7-row sums (rows y_in-6 .. y_in) ----
+      // The change of the squares is one 24-bit product (the pixels are
+      // below 2^18), exact modulo 2^32: for uint16 frames the reference's
+      // wrapped sum, for uint32 frames the sum of the squares' low 26 bits
+      // once the rare rows below take the high parts out.
+      v0w[q] += d0;
+      v0q[q] += static_cast<uint32_t>(
+          __mul24(static_cast<int>(d0), static_cast<int>(p0)));
+      // Stage 2's entering and leaving rows, as a change to its sums.
+      v2w[q] += d2;
+    }
 #pragma unroll
     for (int q = 0; q < KQ; q++)
       ring[slot][q][lane] = static_cast<uint16_t>(val[q]);
 #pragma unroll
     for (int h = 0; h < P; h++)
       mh[h] = (mh[h] << 4) | ((mbits >> (4 * h)) & 0xFu);
-    if (__builtin_expect((recs & (rec_bit(kRecW, 5) | rec_bit(kRecW, 16))) != 0, 0)) {
-      if (recs & rec_bit(kRecW, 5))
-        add_high<KQ>(high[s5][lane], in2);
-      if (recs & rec_bit(kRecW, 16))
-        add_high<KQ>(high[s16][lane], out2);
+    if (kU32 && __builtin_expect((recs & (rec_bit(kRecW, 5) | rec_bit(kRecW, 16) |
+                                           rec_bit(kRecH, 0) | rec_bit(kRecH, 7))) != 0,
+                                 0)) {
+      // Rows with a pixel at or above 2^16 (before row y_in is in, row
+      // y_in - k is bit k of the records): the high ring's bits 16-17 of
+      // stage 2's entering and leaving rows.
+      if (recs & (rec_bit(kRecW, 5) | rec_bit(kRecW, 16))) {
+        uint32_t hin[KQ] = {}, hout[KQ] = {};
+        if (recs & rec_bit(kRecW, 5))
+          add_high<KQ>(high[s5][lane], hin);
+        if (recs & rec_bit(kRecW, 16))
+          add_high<KQ>(high[s16][lane], hout);
 #pragma unroll
-      for (int q = 0; q < KQ; q++)
-        d2[q] = in2[q] - out2[q];
-    }
-
-    // ---- stage 0: vertical 7-row sums (rows y_in-6 .. y_in) ----
-    // (The change of the squares is formed in either branch and added after
-    // them, so that no sum is stored from two places.)
-    uint32_t dq[KQ];
-    if (!kU32 ||
-        __builtin_expect((recs & (rec_bit(kRecH, 0) | rec_bit(kRecH, 7))) == 0, 1)) {
-      // Every pixel entering or leaving is below 2^13 (uint16 frames: below
-      // 2^16, and the product wraps as the reference's sum does), so the
-      // change of the squares is one 24-bit product.
+        for (int q = 0; q < KQ; q++)
+          v2w[q] += hin[q] - hout[q];
+      }
+      // A pixel of 2^13 or more entering or leaving stage 0: the leaving
+      // row whole, and the squares' parts from bit 26 up moved from the
+      // low sums to the high ones.
+      if (recs & (rec_bit(kRecH, 0) | rec_bit(kRecH, 7))) {
+        uint32_t o[KQ];
 #pragma unroll
-      for (int q = 0; q < KQ; q++)
-        dq[q] = static_cast<uint32_t>(__mul24(static_cast<int>(val[q] - out0[q]),
-                                              static_cast<int>(val[q] + out0[q])));
-    } else {
-      // A pixel of 2^13 or more enters or leaves (uint32 frames only): the
-      // squares split at bit 26.
-      if (recs & rec_bit(kRecW, 7))
-        add_high<KQ>(high[s7][lane], out0);
+        for (int q = 0; q < KQ; q++)
+          o[q] = out0[q];
+        if (recs & rec_bit(kRecW, 7)) {
+          add_high<KQ>(high[s7][lane], o);
 #pragma unroll
-      for (int q = 0; q < KQ; q++) {
-        const uint64_t pi = static_cast<uint64_t>(val[q] & (kFast - 1u)) *
-                            (val[q] & (kFast - 1u));
-        const uint64_t po = static_cast<uint64_t>(out0[q] & (kFast - 1u)) *
-                            (out0[q] & (kFast - 1u));
-        dq[q] = (static_cast<uint32_t>(pi) & (kSplit - 1u)) -
-                (static_cast<uint32_t>(po) & (kSplit - 1u));
-        v0h[q] += static_cast<uint32_t>(pi >> 26) - static_cast<uint32_t>(po >> 26);
+          for (int q = 0; q < KQ; q++) {
+            const uint32_t hi = o[q] - out0[q];
+            v0w[q] -= hi;
+            v0q[q] -= static_cast<uint32_t>(
+                __mul24(static_cast<int>(hi), static_cast<int>(o[q] + out0[q])));
+          }
+        }
+#pragma unroll
+        for (int q = 0; q < KQ; q++) {
+          const uint64_t pi = static_cast<uint64_t>(val[q] & (kFast - 1u)) *
+                              (val[q] & (kFast - 1u));
+          const uint64_t po = static_cast<uint64_t>(o[q] & (kFast - 1u)) *
+                              (o[q] & (kFast - 1u));
+          const uint32_t dh =
+              static_cast<uint32_t>(pi >> 26) - static_cast<uint32_t>(po >> 26);
+          v0h[q] += dh;
+          v0q[q] -= dh << 26;
+        }
       }
     }
-#pragma unroll
-    for (int q = 0; q < KQ; q++) {
-      v0w[q] += val[q] - out0[q];
-      v0q[q] += dq[q];
-    }
-    if (__builtin_expect(wide != 0u, 0))
-      high[slot][lane] = static_cast<typename WaveLds<KQ>::High>(pack_high<KQ>(val));
-
     // ---- stage 0: 7 columns across lanes, test -> scr0 of row y_in-3 ----
     uint32_t scr0 = 0; // starts as the masked bits of row y_in-3
 #pragma unroll
@@ -767,9 +781,11 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
     // is skipped (scr0 the masked pixels, no erode bits), which changes
     // nothing that reaches an output.  A pixel of 2^13 or more in rows
     // y_in-6 .. y_in (uint32 frames) brings the squares' high parts in.
+    // The plan's countdown starts at six, so rows before t = 6 never take
+    // the static path, and the other path skips them.
     bool quiet = true;
-    if (t >= 6) {
-      const bool hwin = kU32 && (recs & (0x7Full << kRecH)) != 0;
+    {
+      const uint64_t hwin = kU32 ? (recs & (0x7Full << kRecH)) : 0;
       if ((pstate & 15u) == 0u) {
         // Masks that stay put (or none): the plan's counts.
         if (__builtin_expect(pstate != 0u, 0)) {
@@ -780,7 +796,7 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
         lattice_sums<3>(v0w, hw, std::make_integer_sequence<int, P>{});
         lattice_sums<3>(v0q, hq, std::make_integer_sequence<int, P>{});
         uint32_t lattice = 0;
-        if (__builtin_expect(!hwin, 1)) {
+        if (__builtin_expect(hwin == 0, 1)) {
 #pragma unroll
           for (int h = 0; h < P; h++) {
             const int c = 4 * h + 2;
@@ -798,12 +814,12 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
               lattice |= 1u << c;
           }
         }
-        if (!s_ok || __ballot((lattice & ~s_pat) != 0) != 0) {
+        if (s_ok == 0u || __ballot((lattice & ~s_pat) != 0) != 0) {
           quiet = false;
           scr0 |= lattice;
           slide_all<3>(v0w, hw, std::make_integer_sequence<int, P>{});
           slide_all<3>(v0q, hq, std::make_integer_sequence<int, P>{});
-          if (__builtin_expect(!hwin, 1)) {
+          if (__builtin_expect(hwin == 0, 1)) {
             if (s_all49) {
               const R root49 = table[49];
 #pragma unroll
@@ -831,7 +847,7 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
             }
           }
         }
-      } else {
+      } else if (t >= 6) {
         // Masks that move: each column's unmasked pixels in rows
         // y_in-6 .. y_in counted from the clear bits of the masked history,
         // carried in the top bits of the sums.
@@ -864,10 +880,9 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
       const uint32_t m3 = scr0_masked;
       uint32_t y = 0, ybit = 0, yubit = 0;
       if (quiet) {
-        if (t >= 6) {
-          y = s_y;
-          ybit = s_yany;
-        }
+        // (Before t = 6 no plan has been made in this sweep: s_y is empty.)
+        y = s_y;
+        ybit = s_yany;
       } else if (__ballot(scr0 != 0) != 0) {
         const uint32_t left = from_left(scr0);
         const uint32_t right = from_right(scr0);
@@ -902,27 +917,37 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
 
     // ---- stage 2: vertical 11-row sums of values outside scr1 ----
     {
-      // The entering row's value less the leaving row's, then the two
-      // corrections, each only where the wave-uniform record says its row
-      // holds an unmasked scr1 pixel: an entering one must not be added, a
-      // leaving one was never added (masked pixels are zero anyway).
-#pragma unroll
-      for (int q = 0; q < KQ; q++)
-        v2w[q] += d2[q];
+      // The entering row's value less the leaving row's is in; the two
+      // corrections follow, each only where the wave-uniform record says
+      // its row holds an unmasked scr1 pixel: an entering one must not be
+      // added, a leaving one was never added (masked pixels are zero
+      // anyway).  Their rows are whole here (high ring bits added).
       if (__builtin_expect((recs & (rec_bit(kRecX, 0) | rec_bit(kRecX, 11))) != 0, 0)) {
         if (recs & rec_bit(kRecX, 0)) {
+          uint32_t w[KQ];
 #pragma unroll
           for (int q = 0; q < KQ; q++)
-            v2w[q] -= ((x1 >> q) & 1u) ? in2[q] : 0u;
+            w[q] = in2[q];
+          if (recs & rec_bit(kRecW, 5))
+            add_high<KQ>(high[s5][lane], w);
+#pragma unroll
+          for (int q = 0; q < KQ; q++)
+            v2w[q] -= ((x1 >> q) & 1u) ? w[q] : 0u;
         }
         if (recs & rec_bit(kRecX, 11)) {
           uint32_t xout = 0; // row y_in-16's exclusion bits
 #pragma unroll
           for (int h = 0; h < P; h++)
             xout |= (static_cast<uint32_t>(xh[h] >> 40) & 0xFu) << (4 * h);
+          uint32_t w[KQ];
 #pragma unroll
           for (int q = 0; q < KQ; q++)
-            v2w[q] += ((xout >> q) & 1u) ? out2[q] : 0u;
+            w[q] = out2[q];
+          if (recs & rec_bit(kRecW, 16))
+            add_high<KQ>(high[s16][lane], w);
+#pragma unroll
+          for (int q = 0; q < KQ; q++)
+            v2w[q] += ((xout >> q) & 1u) ? w[q] : 0u;
         }
       }
 #pragma unroll
@@ -990,6 +1015,10 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
         }
       }
     }
+    // The entering row's high bits take the slot of row y_in - 16 only now,
+    // after stage 2's correction for that row has read them.
+    if (__builtin_expect(wide != 0u, 0))
+      high[slot][lane] = static_cast<typename WaveLds<KQ>::High>(pack_high<KQ>(val));
     slot = (slot + 1) & (kRing - 1);
   };
   static_assert(kAhead == 2, "the buffers rotate over three steps");
