@@ -579,7 +579,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--frames", type=int, default=32,
                     help="frames per batch (one step), resident in HBM")
-    ap.add_argument("--in-flight", type=int, default=1,
+    ap.add_argument("--in-flight", type=int, default=2,
                     help="batch handles queued at once in the timed steps "
                          "(each on its own stream)")
     ap.add_argument("--graph", type=int, default=1,
