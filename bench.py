@@ -10,13 +10,16 @@ with exact per-component moments (2D), all on the device.
 
 Frames are generated on the host, uploaded once and resident in HBM before
 the timed region.  A step is one batch of frames (--frames) through one
-persistent threshold launch, the batched ordering and the batched labelling,
-waited for before the next (--in-flight 2 alternates two batch handles on two
-streams instead); `value` is frames per second over the timed steps.  Beside it: the plain
-per-frame mode (one frame per launch sequence, waiting for each), the fused
-kernel's achieved HBM bandwidth against the 8 TB/s peak from HIP events on
-its stream, and the reference's own CPU threshold (compiled from its
-sources, oracle/_ref) on this host's cores over a bounded sample.  The
+persistent threshold launch, the batched ordering and the batched labelling.
+Two batch handles, each on its own stream, alternate (--in-flight 2, the
+default): a step is queued before the one before it is waited for, and every
+step's results are waited for and checked (--in-flight 1 waits for each step
+before the next).  `value` is frames per second over the timed steps.
+Beside it: the plain per-frame mode (one frame per launch sequence, waiting
+for each), the fused kernel's achieved HBM bandwidth against the 8 TB/s peak
+from HIP events on its stream, and the reference's own CPU threshold
+(compiled from its sources, oracle/_ref) on this host's cores over a bounded
+sample.  The
 `decompress` object measures the step before the path: 16M frames stored as
 bslz4 chunks (the EIGER's bitshuffle-LZ4 HDF5 filter), resident in HBM and
 decoded into frames on the device.  The `narrow` object measures
@@ -629,7 +632,7 @@ def main():
         tuned = sf.autotune(descs, label=not args.no_label)
         log(f"autotune: {tuned}")
 
-    # Batch handles in flight (one by default): each has its own stream and
+    # Batch handles in flight (two by default): each has its own stream and
     # slots, so with two the next step is queued before the last one is
     # waited for; every step's results are still waited for and checked.
     batches = [sf.Batch() for _ in range(max(1, args.in_flight))]

@@ -27,8 +27,8 @@
 //
 // Per row entering at the bottom, the wave
 //   * updates stage 0's vertical 7-row sums of values and of squares per
-//     column (registers), with the square's change as one exact product
-//     (in - out)(in + out);
+//     column (registers), with the square's change as one 24-bit product
+//     (in - out)(in + out) modulo 2^32;
 //   * sums 7 columns across lanes (DPP lane shifts) and tests, finishing scr0
 //     three rows up.  Where no masked pixel lies in any of the wave's windows
 //     (a wave-uniform test) every count is 49; elsewhere the counts come from
@@ -46,11 +46,14 @@
 //
 // Arithmetic.  Sums of values are exact in 32 bits while every valid pixel is
 // below 2^18 (49 * (2^18-1) < 2^26 under a count at bit 26; 121 * (2^18-1) <
-// 2^25 under a count at bit 25); squares below 2^36 and their sums below 2^53
-// are exact doubles.  A wave that meets a larger valid pixel raises the
-// frame's overflow flag and the caller re-runs that frame with the direct
-// kernels, whose sums are in the reference's own wide types.  uint16 frames
-// never overflow.  The tests are the reference's expressions in its real type
+// 2^25 under a count at bit 25).  Sums of squares are 32-bit too: while the
+// window's pixels are below 2^13, 49 squares below 2^26 cannot wrap; a row
+// with a pixel of 2^13 or more (a wave-uniform record says which) moves its
+// squares' parts from bit 26 up into a second sum, and the test rebuilds
+// the exact sum (below 2^53) as a double from the two.  A wave that meets a
+// valid pixel of 2^18 or more raises the frame's overflow flag and the
+// caller re-runs that frame with the direct kernels, whose sums are in the
+// reference's own wide types.  uint16 frames never overflow.  The tests are the reference's expressions in its real type
 // on the exact sums; for uint32 frames every intermediate of the left side is
 // an integer below 2^53, so it is formed with a fused multiply-add that is
 // exact, and the right side keeps the reference's two roundings; the stage-0
