@@ -53,7 +53,7 @@ Engine::Engine(void *borrowed_stream) {
         "uploading the root table");
   check(hipStreamSynchronize(stream_), "uploading the root table");
   desc_.ensure(2 * kDescHalf, stream_);
-  queue_.ensure(256, stream_);
+  queue_.ensure(kQueueBytes, stream_);
 }
 
 Engine::~Engine() {

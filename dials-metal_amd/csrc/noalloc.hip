@@ -70,7 +70,7 @@ Layout layout_for(uint32_t h, uint32_t w, int bits) {
   L.capacity = n * static_cast<size_t>(bits / 8) / sizeof(sf_signal_pixel);
   L.counters = 0;
   L.queue = up(L.counters + sfi::kCounters * sizeof(uint32_t));
-  L.desc = up(L.queue + sizeof(uint32_t));
+  L.desc = up(L.queue + sfi::kQueueBytes);
   L.mask = up(L.desc + sizeof(Descriptors));
   L.prefix = up(L.mask + 4 * L.words);
   L.totals = up(L.prefix + 4 * L.words);

@@ -203,8 +203,9 @@ __global__ __launch_bounds__(256) void set_bits(const FrameDev *__restrict__ fra
 __global__ __launch_bounds__(256) void clear_frames(
     const FrameDev *__restrict__ frames, int masks, int first,
     uint32_t *__restrict__ queue) {
-  if (queue && blockIdx.x == 0 && blockIdx.y == 0 && threadIdx.x == 0)
-    *queue = 0;
+  if (queue && blockIdx.x == 0 && blockIdx.y == 0 &&
+      threadIdx.x < static_cast<unsigned>(sfi::kQueueParts))
+    queue[sfi::kQueueStride * threadIdx.x] = 0;
   const FrameDev f = frames[blockIdx.y];
   if (blockIdx.x == 0 && static_cast<int>(threadIdx.x) >= first &&
       threadIdx.x < sfi::kCounters)

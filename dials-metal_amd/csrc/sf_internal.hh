@@ -178,10 +178,16 @@ FusedParams to_params(const FrameArgs &a, const FusedPlan &plan,
 inline bool fused_narrow(uint32_t width, int lane_columns) {
   return width < static_cast<uint32_t>(lane_columns);
 }
+// The fused launch's item counters: one per XCD of the device, each in its
+// own 256-byte line (kQueueStride words apart), kQueueBytes in all.
+constexpr int kQueueParts = 8;
+constexpr int kQueueStride = 64;
+constexpr size_t kQueueBytes = kQueueParts * kQueueStride * sizeof(uint32_t);
+
 // One persistent launch over every item of `nframes` frames (descriptors in
 // device memory, planned with `lane_columns`), mixed sizes and pixel types;
 // each frame's mask and overflow word cleared beforehand, and `queue`, the
-// device word from which waves take items, zeroed.
+// kQueueBytes of counters from which waves take items, zeroed.
 void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
                         uint32_t total_items, const float *tf, const double *td,
                         uint32_t *queue, int lane_columns, int waves_per_block,

@@ -73,6 +73,7 @@ constexpr int kHalo = 10;               // columns of halo each side: 3 + 2 + 5
 constexpr int kAhead = 2;               // rows a wave keeps in flight from HBM
 constexpr int kRing = 16;               // raw rows held: entering .. 15 above
 constexpr uint32_t kStage = 48;         // candidates staged per wave
+constexpr uint32_t kChunk = 8;          // consecutive items dealt together
 constexpr uint32_t kFast = 1u << 18;    // fast-path bound on valid pixels
 constexpr uint32_t kWide = 1u << 16;    // pixels at or above this use the high ring
 constexpr int kTable = 64;              // root table entries used (m <= 49)
@@ -1038,11 +1039,13 @@ __device__ __forceinline__ void sweep(const FusedParams &a, uint32_t item,
                           __HIP_MEMORY_SCOPE_AGENT);
 }
 
-// Every wave takes items from a queue (one atomic a wave per item), over all
-// the frames of a batch (item_base ascending), any mix of sizes and pixel
-// types in one launch: items of unequal cost then finish within about one
-// item of each other.  The grid is the device's resident waves, so the launch
-// is persistent, and every wave leaves when the queue runs past the items.
+// Every wave takes items from its partition's counter (one atomic a wave per
+// item after its first), over all the frames of a batch (item_base
+// ascending), any mix of sizes and pixel types in one launch: items of
+// unequal cost then finish within about one item of each other in each
+// partition, and the partitions hold like shares of every frame.  The grid
+// is the device's resident waves, so the launch is persistent, and every
+// wave leaves when its partition runs past the items.
 template <int Waves, int KQ>
 __global__ __launch_bounds__(Waves *kLanes)
 __attribute__((amdgpu_waves_per_eu(KQ == 4 ? 4 : 2))) void fused_kernel(
@@ -1059,12 +1062,26 @@ __attribute__((amdgpu_waves_per_eu(KQ == 4 ? 4 : 2))) void fused_kernel(
   __syncthreads();
   const uint32_t wave = __builtin_amdgcn_readfirstlane(threadIdx.x / kLanes);
   const uint32_t lane = threadIdx.x % kLanes;
+  // Items come in chunks of kChunk consecutive ones (neighbouring strips of
+  // one band of rows, which share their halo columns through the cache),
+  // dealt round robin over `parts` partitions: blocks b = x mod parts take
+  // partition x's chunks from its own counter, in a line of its own.  The
+  // device deals blocks round robin over its XCDs, so a partition is an
+  // XCD: its waves share one L2, and no counter is hit by the whole device
+  // at once (atomics on one address from every XCD serialise at the memory
+  // side, about 15 ns each).  A wave's first item is its place among its
+  // partition's waves, taken without an atomic; the counter hands out the
+  // rest.  A partition's items rise with its local index, so the first one
+  // past the end ends the wave.  Every partition has a block (b = x), and
+  // every block runs until its partition is done, so no item is missed.
+  const uint32_t parts = min(gridDim.x, static_cast<uint32_t>(sfi::kQueueParts));
+  const uint32_t x = blockIdx.x % parts;
+  const uint32_t part_waves = ((gridDim.x - x + parts - 1) / parts) * Waves;
+  SF_GLOBAL uint32_t *counter = gmem(queue) + sfi::kQueueStride * x;
+  uint32_t local = (blockIdx.x / parts) * Waves + wave;
   for (;;) {
-    uint32_t item = 0;
-    if (lane == 0)
-      item = __hip_atomic_fetch_add(gmem(queue), 1u, __ATOMIC_RELAXED,
-                                    __HIP_MEMORY_SCOPE_AGENT);
-    item = __builtin_amdgcn_readfirstlane(item);
+    const uint32_t item =
+        ((local / kChunk) * parts + x) * kChunk + local % kChunk;
     if (item >= total_items)
       break;
     uint32_t lo = 0, hi = nframes;
@@ -1081,6 +1098,11 @@ __attribute__((amdgpu_waves_per_eu(KQ == 4 ? 4 : 2))) void fused_kernel(
       sweep<uint16_t, KQ>(p, item - p.item_base, lane, lds[wave], table_f);
     else
       sweep<uint32_t, KQ>(p, item - p.item_base, lane, lds[wave], table_d);
+    uint32_t next = 0;
+    if (lane == 0)
+      next = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_RELAXED,
+                                    __HIP_MEMORY_SCOPE_AGENT);
+    local = __builtin_amdgcn_readfirstlane(next) + part_waves;
   }
 }
 
