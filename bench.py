@@ -49,17 +49,19 @@ TRAFFIC = os.path.join(ROOT, "profiles", "fused_kernel_16m_batch32_traffic.json"
 
 def measured_traffic(frames, tuning):
     """HBM bytes per launch of the fused kernel from the committed rocprofv3
-    PMC measurement (FETCH_SIZE and WRITE_SIZE scaled by a device copy of
-    known size), when it was taken at this batch size and launch shape."""
+    PMC measurements (FETCH_SIZE and WRITE_SIZE scaled by a device copy of
+    known size), when one was taken at this batch size and launch shape."""
     try:
         with open(TRAFFIC) as fh:
             m = json.load(fh)
     except (OSError, ValueError):
         return None, None
-    if m.get("frames_per_launch") != frames or any(
-            tuning.get(k) != v for k, v in m.get("tuning", {}).items()):
+    if m.get("frames_per_launch") != frames:
         return None, None
-    return m["traffic_bytes_per_launch"], "profiles/" + os.path.basename(TRAFFIC)
+    for e in m.get("measurements", []):
+        if all(tuning.get(k) == v for k, v in e["tuning"].items()):
+            return e["traffic_bytes_per_launch"], "profiles/" + os.path.basename(TRAFFIC)
+    return None, None
 
 
 def log(*a):
