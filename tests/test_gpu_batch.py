@@ -69,6 +69,41 @@ def run_batch(device, batch, frames, graph):
     return dev
 
 
+def test_item_partitions_cover_every_item(device, oracle_lib):
+    """Batches whose fused item counts fall on and around the launch's
+    partitioning (chunks of eight items dealt over up to eight counters):
+    fewer items than partitions, less than a chunk a partition, ragged last
+    chunks, 40-row and automatic segments; every frame's list equal to the
+    oracle's."""
+    rng = np.random.default_rng(5)
+    frames = []
+    for k in range(11):
+        h, w = int(rng.integers(20, 300)), int(rng.integers(240, 1300))
+        f = rng.poisson(1.5, (h, w)).astype(np.int64)
+        n = h * w // 400
+        f[rng.integers(0, h, n), rng.integers(0, w, n)] += rng.integers(30, 3000, n)
+        f[rng.uniform(size=(h, w)) < 0.01] = 0xFFFFFFFF
+        frames.append(f.astype(np.uint32))
+    want = [oracle_lib.dext(f)[1] for f in frames]
+    dev = [device.DeviceArray.from_numpy(f) for f in frames]
+    start = device.get_tuning()
+    try:
+        for seg in (40, 0):
+            device.set_tuning(seg_rows=seg)
+            batch = device.Batch()
+            for n in (1, 2, 3, 5, 8, 11):
+                say("partitions", seg, n)
+                batch.threshold([(dev[i].ptr, f.shape[0], f.shape[1], 32)
+                                 for i, f in enumerate(frames[:n])])
+                counts, status = batch.sync()
+                for i in range(n):
+                    assert status[i] == 0, (seg, n, i)
+                    assert batch.fetch(i, counts[i]).tobytes() == want[i].tobytes(), (seg, n, i)
+            batch.close()
+    finally:
+        device.set_tuning(**start)
+
+
 @pytest.mark.parametrize("graph,kernel", [(False, "fused"), (True, "fused"),
                                           (True, "direct")])
 def test_mixed_batch_bit_exact(device, synth_cases, matrix, graph, kernel):
