@@ -229,6 +229,20 @@ __global__ __launch_bounds__(256) void fill_words(uint32_t *p, size_t n,
     p[i] = value;
 }
 
+// Copies `words` 16-byte words, then `tail` (< 16) bytes, from pinned host
+// memory mapped into the device's address space.  Every load crosses PCIe,
+// so the grid only needs enough loads in flight to cover its latency.
+__global__ __launch_bounds__(256) void pull_words(const uint4 *__restrict__ src,
+                                                  uint4 *__restrict__ dst,
+                                                  size_t words, uint32_t tail) {
+  for (size_t i = blockIdx.x * static_cast<size_t>(blockDim.x) + threadIdx.x;
+       i < words; i += static_cast<size_t>(gridDim.x) * blockDim.x)
+    dst[i] = src[i];
+  if (blockIdx.x == 0 && threadIdx.x < tail)
+    reinterpret_cast<uint8_t *>(dst + words)[threadIdx.x] =
+        reinterpret_cast<const uint8_t *>(src + words)[threadIdx.x];
+}
+
 } // namespace
 
 namespace sfi {
@@ -286,6 +300,19 @@ void launch_fill(uint32_t *p, size_t n, uint32_t value, hipStream_t stream) {
     blocks = 4096;
   fill_words<<<static_cast<unsigned>(blocks), 256, 0, stream>>>(p, n, value);
   check(hipGetLastError(), "launching a fill");
+}
+
+void launch_pull(const void *mapped_src, void *dst, size_t bytes,
+                 hipStream_t stream) {
+  if (bytes == 0)
+    return;
+  // 256 blocks of 256 threads: 64K loads of 16 bytes in flight, which
+  // measures at the link's rate (57 GB/s for a 72 MB frame); more blocks
+  // only add contention.
+  pull_words<<<256, 256, 0, stream>>>(static_cast<const uint4 *>(mapped_src),
+                                      static_cast<uint4 *>(dst), bytes / 16,
+                                      static_cast<uint32_t>(bytes % 16));
+  check(hipGetLastError(), "launching a host pull");
 }
 
 } // namespace sfi

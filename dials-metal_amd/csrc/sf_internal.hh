@@ -197,6 +197,12 @@ void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
 // correct when the sequence is captured into a graph and replayed.
 void launch_fill(uint32_t *p, size_t n, uint32_t value, hipStream_t stream);
 
+// Copies `bytes` from pinned host memory, through its device-side mapping
+// `mapped_src`, to device memory `dst`, with a kernel; both addresses are
+// 16-byte aligned.
+void launch_pull(const void *mapped_src, void *dst, size_t bytes,
+                 hipStream_t stream);
+
 // One frame's bit array to scan: the exclusive prefix of per-word popcounts
 // (frame-local, from zero).  With `live` (a device count of bits in use,
 // refused as zero above `capacity`), words past the live bits read as zero.

@@ -73,6 +73,37 @@ def test_edge_shapes_against_oracle(device, kernel, oracle_lib):
             assert rc == 0 and got.tobytes() == want.tobytes(), (h, w, dt)
 
 
+def test_pinned_frames_against_oracle(device, kernel, oracle_lib):
+    """Frames in pinned host memory (sf_host_alloc), which find pulls with a
+    kernel over the link: at the allocation's start, 16 bytes in, and 4 bytes
+    in (not 16-byte aligned: the runtime's copy), with byte counts that leave
+    a tail of under 16 bytes, each the oracle's list."""
+    import ctypes
+    rng = np.random.default_rng(123)
+    room = 700 * 465 * 4 + 64
+    p = device.lib().sf_host_alloc(room)
+    assert p
+    try:
+        raw = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint8)),
+                                    shape=(room,))
+        for (h, w) in [(1, 1), (11, 233), (233, 11), (64, 257), (700, 465)]:
+            for dt in (np.uint16, np.uint32):
+                for offset in (0, 16, 4):
+                    f = rng.poisson(3.0, (h, w)).astype(np.int64)
+                    s = rng.uniform(size=(h, w)) < 0.03
+                    f[s] += rng.integers(50, 5000, s.sum())
+                    f[rng.uniform(size=(h, w)) < 0.01] = np.iinfo(dt).max
+                    f = f.astype(dt)
+                    view = raw[offset:offset + f.nbytes].view(dt).reshape(h, w)
+                    view[...] = f
+                    say(kernel, "pinned", h, w, dt, offset)
+                    rc, got = device.find(view)
+                    _, want = oracle_lib.dext(f)
+                    assert rc == 0 and got.tobytes() == want.tobytes(), (h, w, dt, offset)
+    finally:
+        device.lib().sf_host_free(p)
+
+
 def test_all_masked_and_empty(device, kernel):
     for dt in (np.uint16, np.uint32):
         top = np.iinfo(dt).max
