@@ -21,6 +21,7 @@
 #include <vector>
 
 #include "engine.hh"
+#include "host_copy.hh"
 #include "stitch.hh"
 #include "sf_internal.hh"
 #include "sf_labeller.hh"
@@ -159,9 +160,19 @@ using namespace sfi;
 // previous frame is still there to link the next one to.
 struct sf_workspace {
   explicit sf_workspace(void *stream) : engine(stream) {}
+  ~sf_workspace() {
+    (void)hipStreamSynchronize(engine.stream()); // nothing still reads `stage`
+    for (hipEvent_t e : pulled)
+      if (e)
+        (void)hipEventDestroy(e);
+  }
   Engine engine;
   int cur = 1;
   DeviceBuffer upload; // host frames for sf_find_run
+  // Pageable host frames pass through these two pinned pieces in turn; each
+  // event marks the end of the last kernel that read its piece.
+  Pinned stage[2];
+  hipEvent_t pulled[2] = {nullptr, nullptr};
   FrameSlot &slot() { return engine.slot(cur); }
   FrameSlot &other() { return engine.slot(cur ^ 1); }
   bool has_frame() { return engine.slot_count() > 0 && slot().thresholded; }
@@ -460,6 +471,34 @@ const void *mapped_host(const void *p) {
   return attr.type == hipMemoryTypeHost ? attr.devicePointer : nullptr;
 }
 
+// Copies a pageable host frame to the workspace's upload buffer: the host's
+// copy threads fill one pinned piece while the kernel pulls the other, so the
+// transfer runs at the slower of the host copy and the link, not at the
+// runtime's pageable path (which stages through its own buffers).
+void pull_pageable(sf_workspace *ws, const void *frame, size_t bytes) {
+  constexpr size_t kPiece = size_t{8} << 20;
+  hipStream_t s = ws->engine.stream();
+  const char *src = static_cast<const char *>(frame);
+  char *dst = static_cast<char *>(ws->upload.ptr);
+  for (size_t off = 0, k = 0; off < bytes; off += kPiece, k++) {
+    const size_t n = std::min(kPiece, bytes - off);
+    char *piece = ws->stage[k & 1].get<char>(kPiece);
+    hipEvent_t &done = ws->pulled[k & 1];
+    if (done) // the kernel that last read this piece has finished
+      check(hipEventSynchronize(done), "hipEventSynchronize");
+    else
+      check(hipEventCreateWithFlags(&done, hipEventDisableTiming),
+            "hipEventCreate");
+    const CopyPiece cp{piece, src + off, n};
+    host_copy(&cp, 1);
+    const void *mapped = mapped_host(piece);
+    if (mapped == nullptr)
+      throw device_error("pinned staging has no device mapping");
+    launch_pull(mapped, dst + off, n, s);
+    check(hipEventRecord(done, s), "hipEventRecord");
+  }
+}
+
 int find_run(int pixel_bits, const void *frame, size_t height, size_t width,
              int kernel, bool timing, size_t *found) {
   if (found)
@@ -491,10 +530,12 @@ int find_run(int pixel_bits, const void *frame, size_t height, size_t width,
     const void *mapped = mapped_host(frame);
     if (mapped && reinterpret_cast<uintptr_t>(mapped) % 16 == 0)
       launch_pull(mapped, ws->upload.ptr, bytes, ws->engine.stream());
-    else
+    else if (mapped)
       check(hipMemcpyAsync(ws->upload.ptr, frame, bytes, hipMemcpyHostToDevice,
                            ws->engine.stream()),
             "copying a frame to the device");
+    else
+      pull_pageable(ws.get(), frame, bytes);
     if (timing)
       check(hipEventRecord(up[1], ws->engine.stream()), "hipEventRecord");
     ws->engine.set_timing(timing);
