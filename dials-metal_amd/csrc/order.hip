@@ -302,6 +302,15 @@ void launch_fill(uint32_t *p, size_t n, uint32_t value, hipStream_t stream) {
   check(hipGetLastError(), "launching a fill");
 }
 
+const void *mapped_host(const void *p) {
+  hipPointerAttribute_t attr{};
+  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
+    (void)hipGetLastError(); // pageable memory can report an error here
+    return nullptr;
+  }
+  return attr.type == hipMemoryTypeHost ? attr.devicePointer : nullptr;
+}
+
 void launch_pull(const void *mapped_src, void *dst, size_t bytes,
                  hipStream_t stream) {
   if (bytes == 0)

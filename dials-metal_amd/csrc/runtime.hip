@@ -460,17 +460,6 @@ bool injected_failure_now() {
   return false;
 }
 
-// The device's address for `p` when it is pinned host memory mapped into
-// the device's address space (hipHostMalloc'd), else null (pageable memory).
-const void *mapped_host(const void *p) {
-  hipPointerAttribute_t attr{};
-  if (hipPointerGetAttributes(&attr, p) != hipSuccess) {
-    (void)hipGetLastError(); // pageable memory can report an error here
-    return nullptr;
-  }
-  return attr.type == hipMemoryTypeHost ? attr.devicePointer : nullptr;
-}
-
 // Copies a pageable host frame to the workspace's upload buffer: the host's
 // copy threads fill one pinned piece while the kernel pulls the other, so the
 // transfer runs at the slower of the host copy and the link, not at the

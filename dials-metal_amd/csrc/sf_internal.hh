@@ -197,6 +197,10 @@ void launch_fused_batch(const FusedParams *frames_device, uint32_t nframes,
 // correct when the sequence is captured into a graph and replayed.
 void launch_fill(uint32_t *p, size_t n, uint32_t value, hipStream_t stream);
 
+// The device's address for `p` when it is pinned host memory mapped into
+// the device's address space (hipHostMalloc'd), else null (pageable memory).
+const void *mapped_host(const void *p);
+
 // Copies `bytes` from pinned host memory, through its device-side mapping
 // `mapped_src`, to device memory `dst`, with a kernel; both addresses are
 // 16-byte aligned.
