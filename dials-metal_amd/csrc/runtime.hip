@@ -465,7 +465,7 @@ bool injected_failure_now() {
 // transfer runs at the slower of the host copy and the link, not at the
 // runtime's pageable path (which stages through its own buffers).
 void pull_pageable(sf_workspace *ws, const void *frame, size_t bytes) {
-  constexpr size_t kPiece = size_t{8} << 20;
+  constexpr size_t kPiece = size_t{16} << 20;
   hipStream_t s = ws->engine.stream();
   const char *src = static_cast<const char *>(frame);
   char *dst = static_cast<char *>(ws->upload.ptr);
