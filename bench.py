@@ -306,8 +306,9 @@ def decompress(sf, frames_host, distinct=4, per_step=32, steps=10):
     (bitshuffle's default 8192-byte blocks): `distinct` frames encoded on the
     host, `per_step` chunks loaded once (resident in HBM), then each step
     decodes all of them into their own frames.  Also the rate with the chunks
-    uploaded every step (PCIe included), and the oracle's decoder on one
-    frame on one host thread."""
+    uploaded every step (PCIe included): two decoders in turn, as the serving
+    loop runs them, and one decoder waited for each step; and the oracle's
+    decoder on one frame on one host thread."""
     from spotfinder_hip import codec
     f0 = frames_host[0]
     h, w = f0.shape
@@ -341,7 +342,32 @@ def decompress(sf, frames_host, distinct=4, per_step=32, steps=10):
         dec.load(load)
         dec.run()
         dec.sync()
-    host = per_step * 3 / (time.perf_counter() - t0)
+    host_serial = per_step * 3 / (time.perf_counter() - t0)
+    # The same with two decoders in turn (each into frames of its own), as the
+    # serving loop runs them: one's host staging and upload overlap the other's
+    # decoding; each is waited for, and its statuses checked, before it loads
+    # again.
+    dec2 = sf.Decoder()
+    outs2 = [sf.DeviceArray(f0.nbytes) for _ in range(per_step)]
+    load2 = [ld[:5] + (outs2[i],) for i, ld in enumerate(load)]
+    pair = [(dec, load), (dec2, load2)]
+    n_runs = 8
+    t0 = time.perf_counter()
+    for i in range(n_runs):
+        d, ld = pair[i % 2]
+        if i >= 2 and d.sync() != [0] * per_step:
+            raise RuntimeError("a chunk failed to decode: " + sf.last_error())
+        d.load(ld)
+        d.run()
+    for d, _ in pair:
+        if d.sync() != [0] * per_step:
+            raise RuntimeError("a chunk failed to decode: " + sf.last_error())
+    host = per_step * n_runs / (time.perf_counter() - t0)
+    if not np.array_equal(outs2[1].to_numpy(np.uint32, (h, w)), frames_host[1]):
+        raise RuntimeError("a decoded frame differs from its original")
+    dec2.close()
+    for o in outs2:
+        o.free()
     comp = sum(len(c) for c, *_ in load)
     traffic = traffic_source = None
     try:  # rocprofv3 FETCH_SIZE / WRITE_SIZE, scaled, for this launch size
@@ -371,6 +397,7 @@ def decompress(sf, frames_host, distinct=4, per_step=32, steps=10):
         "compression_ratio": per_step * f0.nbytes / comp,
         "kernel_ms": kms,
         "host_upload_included_frames_per_s": host,
+        "host_upload_included_serial_frames_per_s": host_serial,
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": traffic, "traffic_source": traffic_source,
