@@ -104,6 +104,35 @@ def test_pinned_frames_against_oracle(device, kernel, oracle_lib):
         device.lib().sf_host_free(p)
 
 
+def test_large_odd_frames_pinned_and_pageable(device, oracle_lib):
+    """A 4097 x 4097 uint16 frame (33 570 818 bytes: two 16 MiB staging pieces
+    and a last one ending 2 bytes past a 16-byte word) from pageable memory,
+    and from pinned memory pulled whole, each the oracle's list."""
+    import ctypes
+    rng = np.random.default_rng(321)
+    h = w = 4097
+    f = rng.poisson(1.5, (h, w)).astype(np.int64)
+    s = rng.uniform(size=(h, w)) < 0.002
+    f[s] += rng.integers(50, 40000, s.sum())
+    f[rng.uniform(size=(h, w)) < 0.0005] = 65535
+    f = f.astype(np.uint16)
+    _, want = oracle_lib.dext(f)
+    say("large odd pageable", f.shape, f.dtype, f.nbytes)
+    rc, got = device.find(f)
+    assert rc == 0 and got.tobytes() == want.tobytes()
+    p = device.lib().sf_host_alloc(f.nbytes)
+    assert p
+    try:
+        view = np.ctypeslib.as_array(ctypes.cast(p, ctypes.POINTER(ctypes.c_uint16)),
+                                     shape=f.shape)
+        view[...] = f
+        say("large odd pinned", f.shape, f.dtype, f.nbytes)
+        rc, got = device.find(view)
+        assert rc == 0 and got.tobytes() == want.tobytes()
+    finally:
+        device.lib().sf_host_free(p)
+
+
 def test_all_masked_and_empty(device, kernel):
     for dt in (np.uint16, np.uint32):
         top = np.iinfo(dt).max
